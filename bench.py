@@ -1,0 +1,171 @@
+#!/usr/bin/env python
+"""Benchmark of the MI355X smoke solver's hot path.
+
+Workload (one bench step): the 128^3 smoke plume, 64 forward steps, then the VJP of
+0.5 ||rho_64 - target||^2 with respect to the initial state by a reverse sweep with a checkpoint
+every 8 steps (recompute in between). value = simulated fwd+VJP steps per second over the timed
+bench steps, inputs resident in HBM. Also reported: PCG iterations per solve (forward, recompute,
+adjoint), the HBM roofline of the dominant kernel (HIP-event timing of sampled launches inside
+the timed region), and the FP64 CPU oracle timed on the host cores on a bounded sample.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "diff-solver_amd"))
+
+METRIC = "fwd+VJP steps/s at 128^3 and CG iters/solve to 1e-6: CNN warm-start vs plain"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per cell of one launch of each PCG iteration kernel: unknowns read/write
+# their FP32 vectors and the u8 code once; other cells only read the code (and the direction
+# kernel writes p = 0 there).
+BYTES_UNKNOWN = {"pcg_direction_kernel": 4 + 4 + 1 + 4, "pcg_update_kernel": 4 + 4 + 1 + 8 + 4 + 8}
+BYTES_OTHER = {"pcg_direction_kernel": 1 + 4, "pcg_update_kernel": 1}
+
+
+def host_threads():
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(grid, sim_steps, k, sample_steps):
+    """FP64 oracle (same discretisation and stopping rule) on a bounded sample of the workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from ds_smoke import scenes
+
+    threads = host_threads()
+    O.set_threads(threads)
+    sc = scenes.plume(grid)
+    orc = O.Oracle(sc, tolerance=1e-6)
+    s0, ctl = scenes.initial_state(sc), scenes.control(sc)
+    t0 = time.perf_counter()
+    out = orc.loss_gradient(s0, ctl, sc.params, sample_steps, min(k, sample_steps), sc.target)
+    dt = time.perf_counter() - t0
+    info = out["info"]
+    return {
+        "value": sample_steps / dt, "unit": "fwd+VJP steps/s", "cores": threads, "kind": "port",
+        "sample": f"FP64 oracle, {grid}^3 plume, {sample_steps} forward steps + VJP (checkpoint {min(k, sample_steps)}), "
+                  f"PCG to 1e-6; {dt:.1f} s; forward/adjoint iterations per solve "
+                  f"{info.forward_iterations / max(1, info.forward_solves):.0f}/"
+                  f"{info.adjoint_iterations / max(1, info.adjoint_solves):.0f}",
+    }
+
+
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch from the committed rocprofv3 counter summary, when present."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_plume128_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get("hbm_bytes_per_launch")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2, help="timed bench steps (each a 64-step fwd+VJP trajectory)")
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--sim-steps", type=int, default=64)
+    ap.add_argument("--checkpoint", type=int, default=8)
+    ap.add_argument("--profile-every", type=int, default=8)
+    ap.add_argument("--cpu-sample-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (the hot path is single-GPU here); got --gpus {args.gpus}", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from ds_smoke import abi, scenes
+    from ds_smoke.solver import Context
+
+    sc = scenes.plume(args.grid)
+    ctx = Context(sc.config(tolerance=1e-6), abi.MODE_DIFFERENTIABLE)
+    s0 = ctx.state_from_host(scenes.initial_state(sc))
+    ctl = ctx.control_from_host(scenes.control(sc))
+    params = ctx.allocate_parameters(sc.params)
+    target = ctx.to_device(sc.target)
+    grad = ctx.allocate_state_adjoint()
+
+    def one():
+        with torch.cuda.stream(ctx.stream):
+            for t in (grad["density"], grad["temperature"]) + tuple(grad["velocity"]):
+                t.zero_()
+        return ctx.loss_gradient(s0, ctl, params, args.sim_steps, args.checkpoint, target, initial_adj=grad)
+
+    for _ in range(args.warmup):
+        one()
+    torch.cuda.synchronize()
+    ctx.profile_enable(args.profile_every)
+    infos = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, _, info = one()
+        infos.append(info)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    prof = ctx.profile_read()
+    ctx.profile_enable(0)
+
+    sim_steps = args.steps * args.sim_steps
+    info = infos[-1]
+    per_solve = {
+        "forward": info.forward_iterations / max(1, info.forward_solves),
+        "recompute": info.recompute_iterations / max(1, info.recompute_solves),
+        "adjoint": info.adjoint_iterations / max(1, info.adjoint_solves),
+    }
+    n, nf = ctx.domain.cell_count, ctx.domain.unknown_count
+    kern = {}
+    for name, (launches, samples, avg_ms) in prof.items():
+        b = BYTES_UNKNOWN[name] * nf + BYTES_OTHER[name] * (n - nf)
+        kern[name] = {"launches": launches, "samples": samples, "avg_us": avg_ms * 1e3, "algorithmic_bytes": b,
+                      "gbs": b / (avg_ms * 1e-3) / 1e9 if samples else None}
+    dom = max(kern, key=lambda k: kern[k]["launches"] * (kern[k]["avg_us"] or 0.0)) if kern else None
+    roofline = None
+    if dom:
+        d = kern[dom]
+        roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
+                    "algorithmic_bytes_per_launch": d["algorithmic_bytes"], "avg_launch_us": round(d["avg_us"], 2)}
+    result = {
+        "metric": METRIC,
+        "value": round(sim_steps / elapsed, 4),
+        "unit": "fwd+VJP steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"plume{args.grid}_fwd{args.sim_steps}_vjp_checkpoint{args.checkpoint}",
+                   "grid": args.grid, "sim_steps_per_bench_step": args.sim_steps,
+                   "checkpoint_interval": args.checkpoint, "cg_tolerance": 1e-6, "warm_start": "none",
+                   "pressure_tape": True},
+        "cg_iterations_per_solve": {"plain": {k: round(v, 1) for k, v in per_solve.items()}},
+        "loss": loss,
+        "roofline": roofline,
+        "kernels": {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in kern.items()},
+    }
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = cpu_baseline(args.grid, args.sim_steps, args.checkpoint, args.cpu_sample_steps)
+    print(json.dumps(result))
+
+
+if __name__ == "__main__":
+    main()

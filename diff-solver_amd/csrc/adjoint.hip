@@ -1,0 +1,546 @@
+// adjoint.hip — reverse (VJP) operators of the smoke step on gfx950 (solvers/smoke/operators.cu:
+// 297-302, 326-337, 366-379, 415-439, 483-497, 537-561, 922-969).
+//
+// Adjoint fields are FP32. Scatters that follow a data-dependent trace (the transposes of the
+// trilinear samples) use hardware FP32 atomics; every transpose with a fixed stencil is
+// rewritten as a gather so that it needs no atomics: the projection reverse (p_bar per cell),
+// the divergence reverse and the constraint transpose (one face kernel), and the integration
+// reverse (force adjoint per cell, fused with the buoyancy and source reverses). Parameter
+// adjoints are FP64 block partials reduced in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+
+#include "context.hpp"
+#include "kernels.hpp"
+
+namespace ds {
+
+namespace {
+
+// Reverse of trace_rk2 + final sample (operators.cu:422-438 / 544-560): given the adjoint of the
+// trace end point (pbx, pby, pbz) = -deriv * dt * grad(sample) * out_adj, scatter through the
+// midpoint and start samples of the velocity.
+__device__ __forceinline__ void trace_reverse(const Geo& g, const VBound& b, const Vel3& u, float* const va[3], float sx, float sy,
+                                              float sz, const TraceOut& t, float v1x, float v1y, float v1z) {
+    const Sample m0 = sample_face<0, true>(u.u[0], t.mx, t.my, t.mz, g, b);
+    const Sample m1 = sample_face<1, true>(u.u[1], t.mx, t.my, t.mz, g, b);
+    const Sample m2 = sample_face<2, true>(u.u[2], t.mx, t.my, t.mz, g, b);
+    scatter_face<0>(va[0], t.mx, t.my, t.mz, v1x, g, b);
+    scatter_face<1>(va[1], t.mx, t.my, t.mz, v1y, g, b);
+    scatter_face<2>(va[2], t.mx, t.my, t.mz, v1z, g, b);
+    const float mx = m0.gx * v1x + m1.gx * v1y + m2.gx * v1z;
+    const float my = m0.gy * v1x + m1.gy * v1y + m2.gy * v1z;
+    const float mz = m0.gz * v1x + m1.gz * v1y + m2.gz * v1z;
+    const float s = -0.5f * g.dt;
+    scatter_face<0>(va[0], sx, sy, sz, s * mx, g, b);
+    scatter_face<1>(va[1], sx, sy, sz, s * my, g, b);
+    scatter_face<2>(va[2], sx, sy, sz, s * mz, g, b);
+}
+
+// advect_scalar_vjp_kernel (operators.cu:537-561) for NF fields that share one trace: the
+// velocity adjoint of both fields is summed before it is scattered (the map is linear).
+template <int NF, bool HasSolids>
+__global__ __launch_bounds__(256) void advect_scalar_vjp_kernel(Geo g, const VBound* __restrict__ vbp, const SBound* __restrict__ sb0p,
+                                                                const SBound* __restrict__ sb1p, const uint8_t* __restrict__ mask,
+                                                                const float* __restrict__ src0, const float* __restrict__ src1, Vel3 u,
+                                                                const float* __restrict__ oa0, const float* __restrict__ oa1,
+                                                                float* src_adj0, float* src_adj1, F3 va) {
+    const VBound& vb = *vbp;
+    const SBound& sb0 = *sb0p;
+    const SBound& sb1 = *sb1p;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    if (HasSolids && mask[i] != 0) return;
+    const float a0 = oa0[i];
+    const float a1 = NF == 2 ? oa1[i] : 0.0f;
+    if (a0 == 0.0f && a1 == 0.0f) return;
+    int x, y, z;
+    decode_cell(g, i, x, y, z);
+    const float sx = (x + 0.5f) * g.h, sy = (y + 0.5f) * g.h, sz = (z + 0.5f) * g.h;
+    const TraceOut t = trace_rk2<HasSolids>(sx, sy, sz, u, mask, g, vb);
+    const Sample s0 = sample_scalar<true>(src0, t.px, t.py, t.pz, g, sb0);
+    scatter_scalar(src_adj0, t.px, t.py, t.pz, a0, g, sb0);
+    float gx = s0.gx * a0, gy = s0.gy * a0, gz = s0.gz * a0;
+    if (NF == 2) {
+        const Sample s1 = sample_scalar<true>(src1, t.px, t.py, t.pz, g, sb1);
+        scatter_scalar(src_adj1, t.px, t.py, t.pz, a1, g, sb1);
+        gx += s1.gx * a1;
+        gy += s1.gy * a1;
+        gz += s1.gz * a1;
+    }
+    const float v1x = -(t.dx * g.dt) * gx, v1y = -(t.dy * g.dt) * gy, v1z = -(t.dz * g.dt) * gz;
+    trace_reverse(g, vb, u, va.p, sx, sy, sz, t, v1x, v1y, v1z);
+}
+
+// advect_velocity_vjp_kernel, operators.cu:415-439
+template <bool HasSolids>
+__global__ __launch_bounds__(256) void advect_velocity_vjp_kernel(Geo g, const VBound* __restrict__ bp, const uint8_t* __restrict__ mask,
+                                                                  Vel3 u, CF3 oa, F3 va) {
+    const VBound& b = *bp;
+    const uint32_t fi = xcd_block() * blockDim.x + threadIdx.x;
+    if (fi >= g.foff[3]) return;
+    for_face(g, fi, [&]<int A>(uint32_t local) {
+        const float a = oa.p[A][local];
+        if (a == 0.0f) return;
+        int x, y, z;
+        decode_face_axis<A>(g, local, x, y, z);
+        const float sx = (x + (A == 0 ? 0.0f : 0.5f)) * g.h;
+        const float sy = (y + (A == 1 ? 0.0f : 0.5f)) * g.h;
+        const float sz = (z + (A == 2 ? 0.0f : 0.5f)) * g.h;
+        const TraceOut t = trace_rk2<HasSolids>(sx, sy, sz, u, mask, g, b);
+        const Sample s = sample_face<A, true>(u.u[A], t.px, t.py, t.pz, g, b);
+        scatter_face<A>(va.p[A], t.px, t.py, t.pz, a, g, b);
+        const float v1x = -(t.dx * g.dt) * (s.gx * a), v1y = -(t.dy * g.dt) * (s.gy * a), v1z = -(t.dz * g.dt) * (s.gz * a);
+        trace_reverse(g, b, u, va.p, sx, sy, sz, t, v1x, v1y, v1z);
+    });
+}
+
+template <int A>
+__device__ __forceinline__ bool adjacent_solid(const Geo& g, const uint8_t* __restrict__ mask, int x, int y, int z) {
+    const int coord = A == 0 ? x : A == 1 ? y : z;
+    const int n_a = dim_size(g, A);
+    if (coord < n_a && mask[cidx(g, x, y, z)] != 0) return true;
+    if (coord > 0 && mask[cidx(g, x - (A == 0), y - (A == 1), z - (A == 2))] != 0) return true;
+    return false;
+}
+
+// constrained_boundary, operators.cu:452-458
+template <int A>
+__device__ __forceinline__ bool constrained_boundary(const Geo& g, const VBound& b, int coord) {
+    const int n_a = dim_size(g, A);
+    if (coord != 0 && coord != n_a) return false;
+    const uint32_t mode = coord == n_a ? b.modes[2 * A + 1] : b.modes[2 * A];
+    return mode == 0u || mode == 2u;
+}
+
+// Adjoint of the advected velocity at one face from the projected-velocity adjoint and lambda
+// (projection reverse + pressure_rhs_reverse as a gather, operators.cu:922-933,957-969):
+//   adj = u_bar(f) - (h/dt) * (lambda(c0) - lambda(c1)), lambda = 0 off the unknowns.
+struct ProjAdjIn {
+    const float* ua[3];
+    const float* extra[3];
+    const float* lh;
+    const float* ll;
+};
+
+template <int A>
+__device__ __forceinline__ float projected_face_adjoint(const Geo& g, const ProjAdjIn& in, int x, int y, int z, uint32_t local) {
+    float a = in.ua[A][local];
+    if (in.extra[A]) a += in.extra[A][local];
+    const int coord = A == 0 ? x : A == 1 ? y : z;
+    const int n_a = dim_size(g, A);
+    float l0 = 0.0f, l1 = 0.0f;
+    if (coord > 0) {
+        const uint32_t c0 = cidx(g, x - (A == 0), y - (A == 1), z - (A == 2));
+        l0 = in.lh[c0] + in.ll[c0];
+    }
+    if (coord < n_a) {
+        const uint32_t c1 = cidx(g, x, y, z);
+        l1 = in.lh[c1] + in.ll[c1];
+    }
+    return a - g.h * (l0 - l1) / g.dt;
+}
+
+// Projection reverse with the constraint transpose fused (constrain_velocity_vjp_kernel,
+// operators.cu:483-497, as a gather): constrained, solid-adjacent and periodic-duplicate faces
+// receive 0; face 0 of a periodic axis also receives its duplicate's adjoint.
+// Constrain = false: out += projected adjoint (the per-operator projection VJP).
+template <bool Constrain>
+__global__ __launch_bounds__(256) void projection_vjp_face_kernel(Geo g, const VBound* __restrict__ bp, const uint8_t* __restrict__ mask,
+                                                                  uint32_t has_solids, ProjAdjIn in, F3 out) {
+    const VBound& b = *bp;
+    const uint32_t fi = xcd_block() * blockDim.x + threadIdx.x;
+    if (fi >= g.foff[3]) return;
+    for_face(g, fi, [&]<int A>(uint32_t local) {
+        int x, y, z;
+        decode_face_axis<A>(g, local, x, y, z);
+        if (!Constrain) {
+            out.p[A][local] += projected_face_adjoint<A>(g, in, x, y, z, local);
+            return;
+        }
+        const int coord = A == 0 ? x : A == 1 ? y : z;
+        const int n_a = dim_size(g, A);
+        const bool per = vperiodic(b, A);
+        if (constrained_boundary<A>(g, b, coord) || (has_solids && adjacent_solid<A>(g, mask, x, y, z)) || (per && coord == n_a)) {
+            out.p[A][local] = 0.0f;
+            return;
+        }
+        float a = projected_face_adjoint<A>(g, in, x, y, z, local);
+        if (per && coord == 0) {
+            const int dx = A == 0 ? n_a : x, dy = A == 1 ? n_a : y, dz = A == 2 ? n_a : z;
+            if (!(has_solids && adjacent_solid<A>(g, mask, dx, dy, dz)))
+                a += projected_face_adjoint<A>(g, in, dx, dy, dz, fidx(g, A, dx, dy, dz));
+        }
+        out.p[A][local] = a;
+    });
+}
+
+// constrain_velocity_vjp_kernel (operators.cu:483-497) as a gather: out += C^T in.
+__global__ __launch_bounds__(256) void constrain_vjp_kernel(Geo g, const VBound* __restrict__ bp, const uint8_t* __restrict__ mask,
+                                                            uint32_t has_solids, CF3 in, F3 out) {
+    const VBound& b = *bp;
+    const uint32_t fi = xcd_block() * blockDim.x + threadIdx.x;
+    if (fi >= g.foff[3]) return;
+    for_face(g, fi, [&]<int A>(uint32_t local) {
+        int x, y, z;
+        decode_face_axis<A>(g, local, x, y, z);
+        const int coord = A == 0 ? x : A == 1 ? y : z;
+        const int n_a = dim_size(g, A);
+        const bool per = vperiodic(b, A);
+        if (constrained_boundary<A>(g, b, coord) || (has_solids && adjacent_solid<A>(g, mask, x, y, z)) || (per && coord == n_a))
+            return;
+        float a = in.p[A][local];
+        if (per && coord == 0) {
+            const int dx = A == 0 ? n_a : x, dy = A == 1 ? n_a : y, dz = A == 2 ? n_a : z;
+            if (!(has_solids && adjacent_solid<A>(g, mask, dx, dy, dz))) a += in.p[A][fidx(g, A, dx, dy, dz)];
+        }
+        out.p[A][local] += a;
+    });
+}
+
+// out += in over the concatenated faces (the velocity pass-through of integrate_vjp_kernel,
+// operators.cu:370, and accumulate_kernel, operators.cu:292-295).
+__global__ __launch_bounds__(256) void face_accumulate_kernel(Geo g, CF3 in, F3 out) {
+    const uint32_t fi = xcd_block() * blockDim.x + threadIdx.x;
+    if (fi >= g.foff[3]) return;
+    for_face(g, fi, [&]<int A>(uint32_t local) { out.p[A][local] += in.p[A][local]; });
+}
+
+__global__ __launch_bounds__(256) void cell_accumulate_kernel(uint32_t n, const float* __restrict__ in, float* __restrict__ out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] += in[i];
+}
+
+// Force adjoint of fluid cell i from the forced-velocity adjoint (integrate_vjp_kernel,
+// operators.cu:366-379, as a gather): each face adds dt * u_bar / count to each in-domain fluid
+// cell beside it, count = number of such cells.
+__device__ __forceinline__ void force_gather(const Geo& g, const uint8_t* __restrict__ mask, const CF3& fa, uint32_t i, int x, int y,
+                                             int z, float f[3]) {
+    const int c[3]{x, y, z};
+    const uint32_t stride[3]{1u, (uint32_t)g.nx, (uint32_t)g.nx * (uint32_t)g.ny};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int n_a = dim_size(g, a);
+        const uint32_t lo = fidx(g, a, x, y, z);
+        const uint32_t hi = fidx(g, a, x + (a == 0), y + (a == 1), z + (a == 2));
+        const float cnt_lo = 1.0f + ((c[a] > 0 && mask[i - stride[a]] == 0) ? 1.0f : 0.0f);
+        const float cnt_hi = 1.0f + ((c[a] + 1 < n_a && mask[i + stride[a]] == 0) ? 1.0f : 0.0f);
+        f[a] = g.dt * fa.p[a][lo] / cnt_lo + g.dt * fa.p[a][hi] / cnt_hi;
+    }
+}
+
+// Integration reverse (force part), buoyancy reverse (operators.cu:326-337) and source reverse
+// (operators.cu:297-302) fused per fluid cell. Parameter adjoints: FP64 block partials.
+__global__ __launch_bounds__(256) void force_source_vjp_kernel(Geo g, const uint8_t* __restrict__ mask, CF3 forced_adj,
+                                                               const float* __restrict__ rho_s, const float* __restrict__ t_s,
+                                                               const float* __restrict__ rho_s_adj, const float* __restrict__ t_s_adj,
+                                                               Params prm, F3 e_adj, float* __restrict__ prev_rho,
+                                                               float* __restrict__ prev_t, float* __restrict__ s_rho_adj,
+                                                               float* __restrict__ s_t_adj, double* __restrict__ partials) {
+    __shared__ double lds[3 * kMaxWaves];
+    const uint32_t lb = xcd_block();
+    uint32_t chunk = (g.n + gridDim.x - 1) / gridDim.x;
+    chunk = (chunk + 255u) & ~255u;
+    const uint32_t start = min(g.n, lb * chunk), end = min(g.n, start + chunk);
+    const float amb = *prm.ambient, ar = *prm.alpha_rho, at = *prm.alpha_t;
+    double p_amb = 0.0, p_ar = 0.0, p_at = 0.0;
+    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+        if (mask[i] != 0) continue;
+        int x, y, z;
+        decode_cell(g, i, x, y, z);
+        float f[3];
+        force_gather(g, mask, forced_adj, i, x, y, z, f);
+        if (e_adj.p[0]) {
+            e_adj.p[0][i] += f[0];
+            e_adj.p[1][i] += f[1];
+            e_adj.p[2][i] += f[2];
+        }
+        const float rsa = (rho_s_adj ? rho_s_adj[i] : 0.0f) + ar * f[1];
+        const float tsa = (t_s_adj ? t_s_adj[i] : 0.0f) + at * f[1];
+        p_amb += -(double)at * f[1];
+        p_ar += (double)rho_s[i] * f[1];
+        p_at += (double)(t_s[i] - amb) * f[1];
+        prev_rho[i] += rsa;
+        prev_t[i] += tsa;
+        if (s_rho_adj) s_rho_adj[i] += g.dt * rsa;
+        if (s_t_adj) s_t_adj[i] += g.dt * tsa;
+    }
+    double v[3]{p_amb, p_ar, p_at};
+    block_sum<3>(v, lds);
+    if (threadIdx.x == 0 && partials) {
+        partials[0 * gridDim.x + lb] = v[0];
+        partials[1 * gridDim.x + lb] = v[1];
+        partials[2 * gridDim.x + lb] = v[2];
+    }
+}
+
+// buoyancy_vjp_kernel (operators.cu:326-337) with FP64 block partials for the parameters.
+__global__ __launch_bounds__(256) void buoyancy_vjp_kernel(Geo g, const uint8_t* __restrict__ mask, const float* __restrict__ rho,
+                                                           const float* __restrict__ T, Params prm, CF3 fa, float* __restrict__ rho_adj,
+                                                           float* __restrict__ t_adj, F3 e_adj, double* __restrict__ partials) {
+    __shared__ double lds[3 * kMaxWaves];
+    const uint32_t lb = xcd_block();
+    uint32_t chunk = (g.n + gridDim.x - 1) / gridDim.x;
+    chunk = (chunk + 255u) & ~255u;
+    const uint32_t start = min(g.n, lb * chunk), end = min(g.n, start + chunk);
+    const float amb = *prm.ambient, ar = *prm.alpha_rho, at = *prm.alpha_t;
+    double p_amb = 0.0, p_ar = 0.0, p_at = 0.0;
+    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+        if (mask[i] != 0) continue;
+        const float fy = fa.p[1][i];
+        if (e_adj.p[0]) {
+            e_adj.p[0][i] += fa.p[0][i];
+            e_adj.p[1][i] += fy;
+            e_adj.p[2][i] += fa.p[2][i];
+        }
+        rho_adj[i] += ar * fy;
+        t_adj[i] += at * fy;
+        p_amb += -(double)at * fy;
+        p_ar += (double)rho[i] * fy;
+        p_at += (double)(T[i] - amb) * fy;
+    }
+    double v[3]{p_amb, p_ar, p_at};
+    block_sum<3>(v, lds);
+    if (threadIdx.x == 0) {
+        partials[0 * gridDim.x + lb] = v[0];
+        partials[1 * gridDim.x + lb] = v[1];
+        partials[2 * gridDim.x + lb] = v[2];
+    }
+}
+
+// One block: param_adj[k] += sum of the partials of parameter k (fixed order).
+__global__ __launch_bounds__(256) void param_finalize_kernel(const double* __restrict__ partials, uint32_t G, double* amb,
+                                                             double* ar, double* at) {
+    __shared__ double lds[3 * kMaxWaves];
+    const double* parts[3]{partials, partials + G, partials + 2 * G};
+    double red[3];
+    reduce_partials<3>(parts, G, red, lds);
+    if (threadIdx.x == 0) {
+        if (amb) *amb += red[0];
+        if (ar) *ar += red[1];
+        if (at) *at += red[2];
+    }
+}
+
+// source_vjp_kernel, operators.cu:297-302
+__global__ __launch_bounds__(256) void source_vjp_kernel(Geo g, const uint8_t* __restrict__ mask, const float* __restrict__ oa,
+                                                         float* __restrict__ sa, float* __restrict__ srca) {
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= g.n || mask[i] != 0) return;
+    sa[i] += oa[i];
+    if (srca) srca[i] += g.dt * oa[i];
+}
+
+// Force part of integrate_vjp_kernel as a gather (per-operator entry point).
+__global__ __launch_bounds__(256) void integrate_force_vjp_kernel(Geo g, const uint8_t* __restrict__ mask, CF3 fa, F3 force_adj) {
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= g.n || mask[i] != 0) return;
+    int x, y, z;
+    decode_cell(g, i, x, y, z);
+    float f[3];
+    force_gather(g, mask, fa, i, x, y, z, f);
+    force_adj.p[0][i] += f[0];
+    force_adj.p[1][i] += f[1];
+    force_adj.p[2][i] += f[2];
+}
+
+}  // namespace
+
+void launch_advect_scalar_vjp(ds_context* c, int nf, int first_field, const float* src0, const float* src1, CF3 u, const float* oa0,
+                              const float* oa1, float* sa0, float* sa1, F3 va) {
+    const Vel3 v{{u.p[0], u.p[1], u.p[2]}};
+    const uint32_t G = xcd_blocks_for(c->geo.n);
+    if (nf == 2) {
+        if (c->has_solids)
+            advect_scalar_vjp_kernel<2, true><<<G, kBlock, 0, c->stream>>>(c->geo, c->d_vb, c->d_db, c->d_tb, c->mask, src0, src1, v, oa0, oa1, sa0,
+                                                                          sa1, va);
+        else
+            advect_scalar_vjp_kernel<2, false><<<G, kBlock, 0, c->stream>>>(c->geo, c->d_vb, c->d_db, c->d_tb, c->mask, src0, src1, v, oa0, oa1,
+                                                                           sa0, sa1, va);
+    } else {
+        const SBound* b0 = first_field == 0 ? c->d_db : c->d_tb;
+        if (c->has_solids)
+            advect_scalar_vjp_kernel<1, true><<<G, kBlock, 0, c->stream>>>(c->geo, c->d_vb, b0, b0, c->mask, src0, src0, v, oa0, oa0, sa0, sa0,
+                                                                          va);
+        else
+            advect_scalar_vjp_kernel<1, false><<<G, kBlock, 0, c->stream>>>(c->geo, c->d_vb, b0, b0, c->mask, src0, src0, v, oa0, oa0, sa0,
+                                                                           sa0, va);
+    }
+    check_launch("advect_scalar_vjp_kernel");
+}
+
+void launch_advect_velocity_vjp(ds_context* c, CF3 u, CF3 oa, F3 va) {
+    const Vel3 v{{u.p[0], u.p[1], u.p[2]}};
+    const uint32_t G = xcd_blocks_for(c->geo.foff[3]);
+    if (c->has_solids) advect_velocity_vjp_kernel<true><<<G, kBlock, 0, c->stream>>>(c->geo, c->d_vb, c->mask, v, oa, va);
+    else advect_velocity_vjp_kernel<false><<<G, kBlock, 0, c->stream>>>(c->geo, c->d_vb, c->mask, v, oa, va);
+    check_launch("advect_velocity_vjp_kernel");
+}
+
+void launch_projection_vjp_faces(ds_context* c, const float* const ua[3], const float* const extra[3], const float* lh, const float* ll,
+                                 F3 out, bool constrain) {
+    ProjAdjIn in{};
+    for (int a = 0; a < 3; ++a) {
+        in.ua[a] = ua[a];
+        in.extra[a] = extra ? extra[a] : nullptr;
+    }
+    in.lh = lh;
+    in.ll = ll;
+    const uint32_t G = xcd_blocks_for(c->geo.foff[3]);
+    if (constrain)
+        projection_vjp_face_kernel<true><<<G, kBlock, 0, c->stream>>>(c->geo, c->d_vb, c->mask, c->has_solids ? 1u : 0u, in, out);
+    else
+        projection_vjp_face_kernel<false><<<G, kBlock, 0, c->stream>>>(c->geo, c->d_vb, c->mask, c->has_solids ? 1u : 0u, in, out);
+    check_launch("projection_vjp_face_kernel");
+}
+
+void launch_face_accumulate(ds_context* c, CF3 in, F3 out) {
+    face_accumulate_kernel<<<xcd_blocks_for(c->geo.foff[3]), kBlock, 0, c->stream>>>(c->geo, in, out);
+    check_launch("face_accumulate_kernel");
+}
+
+void launch_cell_accumulate(ds_context* c, const float* in, float* out) {
+    cell_accumulate_kernel<<<blocks_for(c->geo.n), kBlock, 0, c->stream>>>(c->geo.n, in, out);
+    check_launch("cell_accumulate_kernel");
+}
+
+void launch_force_source_vjp(ds_context* c, CF3 forced_adj, const float* rho_s, const float* t_s, const float* rho_s_adj,
+                             const float* t_s_adj, const Params& prm, F3 e_adj, float* prev_rho, float* prev_t, float* s_rho_adj,
+                             float* s_t_adj, const ds_parameter_adjoint* param_adj) {
+    force_source_vjp_kernel<<<kPcgBlocks, kBlock, 0, c->stream>>>(c->geo, c->mask, forced_adj, rho_s, t_s, rho_s_adj, t_s_adj, prm, e_adj,
+                                                                  prev_rho, prev_t, s_rho_adj, s_t_adj, c->param_partials);
+    check_launch("force_source_vjp_kernel");
+    if (param_adj && (param_adj->ambient_temperature || param_adj->density_buoyancy || param_adj->temperature_buoyancy)) {
+        param_finalize_kernel<<<1, kBlock, 0, c->stream>>>(c->param_partials, kPcgBlocks, param_adj->ambient_temperature,
+                                                           param_adj->density_buoyancy, param_adj->temperature_buoyancy);
+        check_launch("param_finalize_kernel");
+    }
+}
+
+}  // namespace ds
+
+using namespace ds;
+
+namespace {
+bool need_differentiable(ds_context* c, const char* name) {
+    if (c->mode != DS_MODE_DIFFERENTIABLE) {
+        set_error(std::string(name) + ": the context was not created in differentiable mode");
+        return false;
+    }
+    return true;
+}
+}  // namespace
+
+extern "C" {
+
+ds_status ds_source_vjp(ds_context* c, const float* out_adj, float* state_adj, float* source_adj) {
+    if (!c || !out_adj || !state_adj) {
+        set_error("ds_source_vjp: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] {
+        source_vjp_kernel<<<xcd_blocks_for(c->geo.n), kBlock, 0, c->stream>>>(c->geo, c->mask, out_adj, state_adj, source_adj);
+        check_launch("source_vjp_kernel");
+    });
+}
+
+ds_status ds_buoyancy_vjp(ds_context* c, const float* density, const float* temperature, const ds_parameters* params,
+                          float* const force_adj[3], float* density_adj, float* temperature_adj, float* e_adj[3],
+                          const ds_parameter_adjoint* param_adj) {
+    if (!c || !density || !temperature || !force_adj || !density_adj || !temperature_adj) {
+        set_error("ds_buoyancy_vjp: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    if (!need_differentiable(c, "ds_buoyancy_vjp")) return DS_ERROR_INVALID_ARGUMENT;
+    return guard([&] {
+        const Params prm = params_of(params);
+        F3 ea{{e_adj ? e_adj[0] : nullptr, e_adj ? e_adj[1] : nullptr, e_adj ? e_adj[2] : nullptr}};
+        buoyancy_vjp_kernel<<<kPcgBlocks, kBlock, 0, c->stream>>>(c->geo, c->mask, density, temperature, prm,
+                                                                  CF3{{force_adj[0], force_adj[1], force_adj[2]}}, density_adj,
+                                                                  temperature_adj, ea, c->param_partials);
+        check_launch("buoyancy_vjp_kernel");
+        if (param_adj) {
+            param_finalize_kernel<<<1, kBlock, 0, c->stream>>>(c->param_partials, kPcgBlocks, param_adj->ambient_temperature,
+                                                               param_adj->density_buoyancy, param_adj->temperature_buoyancy);
+            check_launch("param_finalize_kernel");
+        }
+    });
+}
+
+ds_status ds_integrate_velocity_vjp(ds_context* c, float* const out_adj[3], float* velocity_adj[3], float* force_adj[3]) {
+    if (!c || !out_adj || !velocity_adj) {
+        set_error("ds_integrate_velocity_vjp: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] {
+        const CF3 oa{{out_adj[0], out_adj[1], out_adj[2]}};
+        launch_face_accumulate(c, oa, F3{{velocity_adj[0], velocity_adj[1], velocity_adj[2]}});
+        if (force_adj) {
+            integrate_force_vjp_kernel<<<xcd_blocks_for(c->geo.n), kBlock, 0, c->stream>>>(c->geo, c->mask, oa,
+                                                                                           F3{{force_adj[0], force_adj[1], force_adj[2]}});
+            check_launch("integrate_force_vjp_kernel");
+        }
+    });
+}
+
+ds_status ds_advect_velocity_vjp(ds_context* c, float* const velocity[3], float* const out_adj[3], float* velocity_adj[3]) {
+    if (!c || !velocity || !out_adj || !velocity_adj) {
+        set_error("ds_advect_velocity_vjp: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] {
+        launch_advect_velocity_vjp(c, CF3{{velocity[0], velocity[1], velocity[2]}}, CF3{{out_adj[0], out_adj[1], out_adj[2]}},
+                                   F3{{velocity_adj[0], velocity_adj[1], velocity_adj[2]}});
+    });
+}
+
+ds_status ds_constrain_velocity_vjp(ds_context* c, float* const out_adj[3], float* velocity_adj[3]) {
+    if (!c || !out_adj || !velocity_adj) {
+        set_error("ds_constrain_velocity_vjp: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] {
+        constrain_vjp_kernel<<<xcd_blocks_for(c->geo.foff[3]), kBlock, 0, c->stream>>>(
+            c->geo, c->d_vb, c->mask, c->has_solids ? 1u : 0u, CF3{{out_adj[0], out_adj[1], out_adj[2]}},
+            F3{{velocity_adj[0], velocity_adj[1], velocity_adj[2]}});
+        check_launch("constrain_vjp_kernel");
+    });
+}
+
+ds_status ds_advect_scalar_vjp(ds_context* c, uint32_t field, const float* source, float* const velocity[3], const float* out_adj,
+                               float* source_adj, float* velocity_adj[3]) {
+    if (!c || !source || !velocity || !out_adj || !source_adj || !velocity_adj || field > 1u) {
+        set_error("ds_advect_scalar_vjp: invalid argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] {
+        launch_advect_scalar_vjp(c, 1, (int)field, source, source, CF3{{velocity[0], velocity[1], velocity[2]}}, out_adj, out_adj,
+                                 source_adj, source_adj, F3{{velocity_adj[0], velocity_adj[1], velocity_adj[2]}});
+    });
+}
+
+ds_status ds_projection_vjp(ds_context* c, float* const out_adj[3], float* velocity_adj[3], ds_solve_info* info) {
+    if (!c || !out_adj || !velocity_adj) {
+        set_error("ds_projection_vjp: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    if (!need_differentiable(c, "ds_projection_vjp")) return DS_ERROR_INVALID_ARGUMENT;
+    ds_solve_info si{};
+    const ds_status st = guard([&] {
+        const float* ua[3]{out_adj[0], out_adj[1], out_adj[2]};
+        si = pcg_solve_kind(c, 2, ua, nullptr, c->lambda_hi, c->lambda_lo, nullptr, nullptr, c->tolerance, c->max_iterations);
+        c->lambda_valid = true;
+        launch_projection_vjp_faces(c, ua, nullptr, c->lambda_hi, c->lambda_lo, F3{{velocity_adj[0], velocity_adj[1], velocity_adj[2]}},
+                                    false);
+    });
+    if (info) *info = si;
+    if (st != DS_OK) return st;
+    if (!si.converged) {
+        set_error("ds_projection_vjp: adjoint pressure solve did not converge");
+        return std::isfinite(si.final_residual) ? DS_ERROR_NOT_CONVERGED : DS_ERROR_NON_FINITE;
+    }
+    return DS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,213 @@
+// context.hpp — host-side execution context of the smoke solver (ExecutionContext + Resource of
+// the reference: solvers/smoke/model.ixx:15-49, solvers/cuda.ixx:7-28) and the helpers every
+// C-ABI entry point shares: status codes with a thread-local message, launch checks, scratch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <exception>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ds/ds_smoke.h"
+#include "device.hpp"
+
+namespace ds {
+
+void set_error(const std::string& message);
+
+// Throwing is internal only: every C-ABI function catches and converts to a ds_status.
+struct Failure {
+    ds_status status;
+    std::string message;
+};
+
+#define DS_HIP(call)                                                                                  \
+    do {                                                                                              \
+        const hipError_t ds_hip_status_ = (call);                                                     \
+        if (ds_hip_status_ != hipSuccess)                                                             \
+            throw ::ds::Failure{ds_hip_status_ == hipErrorOutOfMemory ? DS_ERROR_OUT_OF_MEMORY : DS_ERROR_RUNTIME, \
+                                std::string(#call) + ": " + hipGetErrorString(ds_hip_status_)};       \
+    } while (0)
+
+// check_launch (operators.cu:28-30): a failed launch becomes DS_ERROR_LAUNCH with the kernel name.
+void check_launch(const char* name);
+
+// Runs body; converts a Failure (or a C++ exception) to a status code plus ds_last_error().
+template <typename F>
+ds_status guard(F&& body) {
+    try {
+        body();
+        return DS_OK;
+    } catch (const Failure& f) {
+        set_error(f.message);
+        return f.status;
+    } catch (const std::bad_alloc&) {
+        set_error("host allocation failed");
+        return DS_ERROR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        set_error(e.what());
+        return DS_ERROR_RUNTIME;
+    }
+}
+
+inline void require(bool ok, const char* message) {
+    if (!ok) throw Failure{DS_ERROR_INVALID_ARGUMENT, message};
+}
+
+inline uint32_t blocks_for(uint64_t count, uint32_t block = kBlock) { return (uint32_t)((count + block - 1) / block); }
+// Block count rounded up to a multiple of 8 for kernels that remap blocks per XCD.
+inline uint32_t xcd_blocks_for(uint64_t count, uint32_t block = kBlock) { return (blocks_for(count, block) + 7u) & ~7u; }
+
+// Reduction layout: kernels of kPcgBlocks (256-thread) blocks leave one FP64 partial per block
+// per quantity; consumers finish the sum in a fixed order (deterministic, no atomics).
+constexpr uint32_t kPcgBlocks = 512;
+// PCG iteration kernels: 1024-thread blocks, each over a contiguous range of whole multiples of
+// kPcgUnroll * 1024 cells; the block count (<= kPcgMaxBlocks, a multiple of 8) is per context.
+constexpr uint32_t kPcgThreads = 1024;
+constexpr uint32_t kPcgUnroll = 4;
+constexpr uint32_t kPcgMaxBlocks = 1024;
+
+enum Partial : int { kPartBB = 0, kPartRR = 1, kPartRZ = 2, kPartPQ = 3, kPartCount = 4 };
+
+// Device status of one PCG solve; written by block 0 of the kernel that decides.
+struct PcgStatus {
+    uint32_t stop;          // 1 once converged (or the guess made the residual small enough)
+    uint32_t iterations;    // iterations performed when stop was set
+    uint32_t rejected;
+    uint32_t pad;
+    double bb;              // ||b||^2
+    double rr;              // last reduced ||r||^2 (recursive or true)
+};
+
+// HIP-event timing of sampled PCG kernel launches (kernel 0 = direction, 1 = update).
+struct Profiler {
+    uint32_t every = 0;
+    uint64_t launches[2]{};
+    std::vector<hipEvent_t> pool;   // created lazily, reused
+    size_t used = 0;
+    struct Rec {
+        int kernel;
+        hipEvent_t a, b;
+    };
+    std::vector<Rec> recs;
+    hipEvent_t take() {
+        if (used == pool.size()) {
+            hipEvent_t e = nullptr;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            pool.push_back(e);
+        }
+        return pool[used++];
+    }
+};
+
+// Plane-marching tiling of the PCG iteration kernels: 64 x 16 (x, y) tiles, zc planes per block.
+constexpr int kPlaneTileX = 64;
+constexpr int kPlaneTileY = 16;
+struct PlaneTiling {
+    uint32_t tiles_x, tiles_y, zc;
+};
+
+struct PcgWork {
+    PlaneTiling tiling{1, 1, 4};
+    float* b = nullptr;          // right-hand side (zero off the unknowns)
+    float* r = nullptr;          // residual
+    float* p[2]{};               // search directions, ping-pong between iterations
+    uint32_t blocks = 8;         // blocks of the PCG kernels
+    double* partials = nullptr;  // kPartCount * kPcgMaxBlocks
+    double* hist = nullptr;      // per-iteration scalars: rz[k], pq[k] (2 * (max_it + 2))
+    PcgStatus* status = nullptr; // device
+    PcgStatus* host_status = nullptr;  // pinned, 2 slots (double-buffered batches)
+    hipEvent_t batch_event[2]{};
+};
+
+}  // namespace ds
+
+struct ds_context {
+    ds_config config{};
+    std::vector<ds_collider> colliders;
+    uint32_t mode = DS_MODE_FORWARD;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    int device = 0;
+
+    ds::Geo geo{};
+    ds::VBound vb{};
+    ds::SBound pb{}, db{}, tb{};
+    // device copies read by the kernels (a runtime index into a by-value kernel argument would
+    // move the argument to scratch memory)
+    const ds::VBound* d_vb = nullptr;
+    const ds::SBound* d_db = nullptr;
+    const ds::SBound* d_tb = nullptr;
+    bool has_solids = false;
+    bool has_fixed_pressure = false;
+    bool pressure_periodic[3]{};
+    uint64_t anchor = 0;
+    uint32_t extra_anchors = 0;
+    uint64_t unknowns = 0, solids = 0;
+    double tolerance = 1e-6;
+    uint32_t max_iterations = 1000;
+    uint32_t replacement_interval = 50;
+
+    // domain (device)
+    uint8_t* mask = nullptr;          // 1 = solid
+    uint8_t* code = nullptr;          // diagonal count of an unknown, 0 = not an unknown
+    float* fixed_term = nullptr;      // sum of fixed-value pressure faces; null when none
+    float* collider_velocity[3]{};    // face collider velocity; null when no colliders
+    float* collider_density = nullptr;
+    float* collider_temperature = nullptr;
+
+    // forward scratch (device)
+    float* force[3]{};
+    float* advected[3]{};
+    float* pressure_hi = nullptr;     // solution of the last forward solve (hi + lo)
+    float* pressure_lo = nullptr;
+    bool pressure_valid = false;
+
+    // adjoint scratch (device, differentiable mode)
+    float* sourced_adj[2]{};          // adjoints of rho_s, T_s
+    float* projected_extra[3]{};      // scalar-advection contributions to the projected-velocity adjoint
+    float* forced_adj[3]{};           // adjoint of the forced velocity u*
+    float* raw_adj[3]{};              // adjoint of the raw advected velocity
+    float* lambda_hi = nullptr;       // adjoint pressure solve solution (hi + lo)
+    float* lambda_lo = nullptr;
+    bool lambda_valid = false;
+    double* param_partials = nullptr; // 3 * kPcgBlocks
+
+    ds::PcgWork pcg;
+    ds::Profiler profiler;
+
+    std::vector<void*> allocations;   // everything above, released at destroy
+};
+
+namespace ds {
+
+template <typename T>
+T* device_alloc(ds_context* c, size_t count) {
+    void* p = nullptr;
+    DS_HIP(hipMalloc(&p, count * sizeof(T) > 0 ? count * sizeof(T) : 1));
+    c->allocations.push_back(p);
+    return static_cast<T*>(p);
+}
+
+void fill_zero(ds_context* c, float* p, uint64_t count);
+void fill_zero_d(ds_context* c, double* p, uint64_t count);
+
+// PCG on the context's pressure matrix: solves A x = b where b is already in c->pcg.b.
+// x is the compensated pair (hi, lo). x0 may be null. Returns the solve info.
+ds_solve_info pcg_solve(ds_context* c, float* x_hi, float* x_lo, const float* x0, double tol, uint32_t max_it);
+
+// Solves A x = b with b built from `u` (kind 1: forward divergence of the advected velocity;
+// kind 2: adjoint, u = projected-velocity adjoint plus `extra`) or given in c->pcg.b (kind 0).
+ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], const float* const extra[3], float* xh,
+                             float* xl, const float* x0h, const float* x0l, double tol, uint32_t max_it);
+uint32_t pcg_periodic_bits(const ds_context* c);
+// out = hi + lo (rounded to FP32).
+void combine_pair(ds_context* c, const float* hi, const float* lo, float* out);
+
+// Launch wrappers shared between translation units.
+void launch_apply_matrix(ds_context* c, const float* x, float* y);
+
+}  // namespace ds

@@ -1,0 +1,508 @@
+// pcg.hip — Jacobi-preconditioned conjugate gradients on the smoke solver's pressure matrix.
+//
+// The matrix is the one the reference's red-black Gauss-Seidel iterates on (pressure_neighbor /
+// rbgs_kernel, solvers/smoke/operators.cu:852-902): for an unknown i, A_ii = number of connected
+// fluid neighbours (periodic wrap included) + number of fixed-value faces, A_ij = -1 for each
+// connected neighbour that is itself an unknown; anchors and solids are not unknowns (p = 0).
+// The per-cell connectivity code holds A_ii (0 = not an unknown). Every work vector is zero off
+// the unknowns, so a stencil sums its in-domain neighbours without looking at their codes.
+//
+// Precision: FP32 vectors and stencils; FP64 dot products, norms and true residuals. The
+// solution is a compensated FP32 pair x = hi + lo (the rounding of x alone to FP32 leaves a
+// residual above 1e-6 ||b|| on smooth right-hand sides). Stopping rule ||b - A x|| <= tol ||b||
+// with the true residual recomputed every `replacement_interval` iterations and at exit.
+//
+// Kernels per iteration (grid-stride over contiguous per-block cell ranges, XCD-grouped):
+//   direction: p_k = r/d + beta p_{k-1} built on the fly at the centre and its 6 neighbours,
+//              q = A p_k, partial <p_k, q>         (reads r, p_{k-1}, code; writes p_k)
+//   update:    q = A p_k again, r -= alpha q, x += alpha p_k, partials <r,z>, <r,r>
+//              (reads r, p_k, code, x; writes r, x)
+// Each consumer finishes its producer's per-block FP64 partials in a fixed order, so every
+// block derives bit-identical scalars and the iteration is deterministic without atomics.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+
+#include "context.hpp"
+#include "pcg_plane.hpp"
+
+namespace ds {
+
+namespace {
+
+// Contiguous cell range of this (XCD-remapped) block; the chunk is a whole number of
+// kPcgUnroll * blockDim cells, so every thread runs the same unrolled trip count.
+__device__ __forceinline__ void block_range(uint32_t n, uint32_t& lb, uint32_t& start, uint32_t& end) {
+    lb = xcd_block();
+    const uint32_t step = kPcgUnroll * blockDim.x;
+    uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    chunk = (chunk + step - 1) / step * step;
+    start = min(n, lb * chunk);
+    end = min(n, start + chunk);
+}
+
+// The 6 neighbours of cell i = (x, y, z) as clamped indices and 0/1 weights: an out-of-domain
+// direction (non-periodic axis) points at i itself with weight 0, a periodic axis wraps. All
+// loads stay in bounds and the gather has no branches, so the loads of several cells issue
+// back to back.
+struct Stencil {
+    uint32_t j[6];
+    float w[6];
+};
+
+__device__ __forceinline__ Stencil stencil(const Geo& g, uint32_t per, int x, int y, int z, uint32_t i) {
+    const uint32_t sy = (uint32_t)g.nx, sz = (uint32_t)g.nx * (uint32_t)g.ny;
+    const bool px = per & 1u, py = per & 2u, pz = per & 4u;
+    Stencil s;
+    s.j[0] = x > 0 ? i - 1u : (px ? i + (uint32_t)(g.nx - 1) : i);
+    s.w[0] = (x > 0 || px) ? 1.0f : 0.0f;
+    s.j[1] = x < g.nx - 1 ? i + 1u : (px ? i - (uint32_t)(g.nx - 1) : i);
+    s.w[1] = (x < g.nx - 1 || px) ? 1.0f : 0.0f;
+    s.j[2] = y > 0 ? i - sy : (py ? i + (uint32_t)(g.ny - 1) * sy : i);
+    s.w[2] = (y > 0 || py) ? 1.0f : 0.0f;
+    s.j[3] = y < g.ny - 1 ? i + sy : (py ? i - (uint32_t)(g.ny - 1) * sy : i);
+    s.w[3] = (y < g.ny - 1 || py) ? 1.0f : 0.0f;
+    s.j[4] = z > 0 ? i - sz : (pz ? i + (uint32_t)(g.nz - 1) * sz : i);
+    s.w[4] = (z > 0 || pz) ? 1.0f : 0.0f;
+    s.j[5] = z < g.nz - 1 ? i + sz : (pz ? i - (uint32_t)(g.nz - 1) * sz : i);
+    s.w[5] = (z < g.nz - 1 || pz) ? 1.0f : 0.0f;
+    return s;
+}
+
+// Sum of v over the 6 neighbours of cell i = (x, y, z); periodic axes wrap (bit k of per).
+template <typename T, typename F>
+__device__ __forceinline__ T neighbour_sum(const Geo& g, uint32_t per, int x, int y, int z, uint32_t i, F&& v) {
+    const Stencil s = stencil(g, per, x, y, z, i);
+    T sum = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sum += (T)s.w[k] * v(s.j[k]);
+    return sum;
+}
+
+struct RhsIn {
+    const float* u[3];        // forward: advected velocity; adjoint: projected-velocity adjoint
+    const float* extra[3];    // adjoint only: scalar-advection contributions (may be null)
+};
+
+enum RhsKind : int { kRhsGiven = 0, kRhsForward = 1, kRhsAdjoint = 2 };
+
+// Right-hand side of one cell.
+// forward (pressure_rhs_kernel, operators.cu:875-883): b = -h^2 div(u)/dt + fixed faces;
+// adjoint (project_velocity_reverse_kernel, operators.cu:922-933, as a gather): p_bar_i =
+//   (dt/h) * sum_a [u_bar(i+e_a) - u_bar(i)] over projectable faces (both cells fluid, in domain).
+template <int Kind>
+__device__ __forceinline__ float cell_rhs(const Geo& g, const RhsIn& in, const uint8_t* __restrict__ mask,
+                                          const float* __restrict__ fixed_term, uint32_t i, int x, int y, int z) {
+    const uint32_t ex0 = (uint32_t)g.nx + 1u, ey1 = (uint32_t)g.ny + 1u;
+    const uint32_t f0 = (uint32_t)x + ex0 * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z);
+    const uint32_t f1 = (uint32_t)x + (uint32_t)g.nx * ((uint32_t)y + ey1 * (uint32_t)z);
+    const uint32_t f2 = i;
+    const uint32_t sz = (uint32_t)g.nx * (uint32_t)g.ny;
+    if (Kind == kRhsForward) {
+        const float sum = (in.u[0][f0 + 1u] - in.u[0][f0]) + (in.u[1][f1 + (uint32_t)g.nx] - in.u[1][f1]) +
+                          (in.u[2][f2 + sz] - in.u[2][f2]);
+        const float divergence = sum * g.inv_h;
+        float b = -g.h * g.h * divergence / g.dt;
+        if (fixed_term) b += fixed_term[i];
+        return b;
+    } else {
+        auto ua = [&](int a, uint32_t f) { return in.u[a][f] + (in.extra[a] ? in.extra[a][f] : 0.0f); };
+        float s = 0.0f;
+        if (x + 1 < g.nx && mask[i + 1u] == 0) s += ua(0, f0 + 1u);
+        if (x > 0 && mask[i - 1u] == 0) s -= ua(0, f0);
+        if (y + 1 < g.ny && mask[i + (uint32_t)g.nx] == 0) s += ua(1, f1 + (uint32_t)g.nx);
+        if (y > 0 && mask[i - (uint32_t)g.nx] == 0) s -= ua(1, f1);
+        if (z + 1 < g.nz && mask[i + sz] == 0) s += ua(2, f2 + sz);
+        if (z > 0 && mask[i - sz] == 0) s -= ua(2, f2);
+        return g.dt * s * g.inv_h;
+    }
+}
+
+// Setup: b (computed or given), x = x0 (or 0), r = b - A x0 (FP64), partials ||b||^2, ||r||^2, <r,z>.
+template <int Kind>
+__global__ __launch_bounds__(1024) void pcg_setup_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
+                                                        const uint8_t* __restrict__ mask, RhsIn in,
+                                                        const float* __restrict__ fixed_term, float* __restrict__ b,
+                                                        const float* x0h, const float* x0l, uint32_t x_is_guess,
+                                                        float* xh, float* xl, float* __restrict__ r,
+                                                        double* __restrict__ partials, PcgStatus* status) {
+    __shared__ double lds[3 * kMaxWaves];
+    uint32_t lb, start, end;
+    block_range(g.n, lb, start, end);
+    double bb = 0.0, rr = 0.0, rz = 0.0;
+    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+        const uint32_t d = code[i];
+        int x, y, z;
+        decode_cell(g, i, x, y, z);
+        float bi = 0.0f;
+        if (d != 0u) bi = Kind == kRhsGiven ? b[i] : cell_rhs<Kind>(g, in, mask, fixed_term, i, x, y, z);
+        if (Kind != kRhsGiven || d == 0u) b[i] = bi;
+        float ri = bi;
+        if (x0h != nullptr) {
+            if (d != 0u) {
+                auto xv = [&](uint32_t j) -> double {
+                    return code[j] ? (double)x0h[j] + (x0l ? (double)x0l[j] : 0.0) : 0.0;
+                };
+                const double ax = (double)d * xv(i) - neighbour_sum<double>(g, per, x, y, z, i, xv);
+                ri = (float)((double)bi - ax);
+            }
+            if (!x_is_guess) {
+                xh[i] = d ? x0h[i] : 0.0f;
+                xl[i] = d && x0l ? x0l[i] : 0.0f;
+            }
+        } else {
+            xh[i] = 0.0f;
+            xl[i] = 0.0f;
+        }
+        if (d == 0u) ri = 0.0f;
+        r[i] = ri;
+        const float zi = ri * inv_diag(d);
+        bb += (double)bi * bi;
+        rr += (double)ri * ri;
+        rz += (double)ri * zi;
+    }
+    double v[3]{bb, rr, rz};
+    block_sum<3>(v, lds);
+    if (threadIdx.x == 0) {
+        partials[kPartBB * kPcgMaxBlocks + lb] = v[0];
+        partials[kPartRR * kPcgMaxBlocks + lb] = v[1];
+        partials[kPartRZ * kPcgMaxBlocks + lb] = v[2];
+        if (lb == 0) {
+            status->stop = 0u;
+            status->iterations = 0u;
+            status->rejected = 0u;
+        }
+    }
+}
+
+// True residual r = b - A (x_hi + x_lo) in FP64; replaces the recursive residual.
+__global__ __launch_bounds__(1024) void pcg_true_residual_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
+                                                                const float* __restrict__ b, const float* __restrict__ xh,
+                                                                const float* __restrict__ xl, float* __restrict__ r,
+                                                                double* __restrict__ partials, PcgStatus* status,
+                                                                uint32_t respect_stop) {
+    __shared__ double lds[3 * kMaxWaves];
+    if (respect_stop && status->stop) return;
+    uint32_t lb, start, end;
+    block_range(g.n, lb, start, end);
+    const uint32_t G = gridDim.x;
+    double rr = 0.0, rz = 0.0;
+    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+        const uint32_t d = code[i];
+        if (d == 0u) continue;
+        int x, y, z;
+        decode_cell(g, i, x, y, z);
+        auto xv = [&](uint32_t j) -> double { return (double)xh[j] + (double)xl[j]; };
+        const double ax = (double)d * xv(i) - neighbour_sum<double>(g, per, x, y, z, i, xv);
+        const double ri = (double)b[i] - ax;
+        const float rf = (float)ri;
+        r[i] = rf;
+        rr += ri * ri;
+        rz += (double)rf * (rf * inv_diag(d));
+    }
+    double v[2]{rr, rz};
+    block_sum<2>(v, lds);
+    if (threadIdx.x == 0) {
+        partials[kPartRR * kPcgMaxBlocks + lb] = v[0];
+        partials[kPartRZ * kPcgMaxBlocks + lb] = v[1];
+    }
+}
+
+// One block: reduce the ||b||^2 (optional) and ||r||^2 partials into the status.
+__global__ __launch_bounds__(256) void pcg_finalize_kernel(const double* __restrict__ partials, uint32_t G,
+                                                           PcgStatus* status, uint32_t with_bb) {
+    __shared__ double lds[3 * kMaxWaves];
+    const double* parts[2]{partials + kPartBB * kPcgMaxBlocks, partials + kPartRR * kPcgMaxBlocks};
+    double red[2];
+    reduce_partials<2>(parts, G, red, lds);
+    if (threadIdx.x == 0) {
+        if (with_bb) status->bb = red[0];
+        status->rr = red[1];
+    }
+}
+
+__global__ void pcg_clear_stop_kernel(PcgStatus* status) {
+    if (threadIdx.x == 0) status->stop = 0u;
+}
+
+__global__ __launch_bounds__(256) void apply_matrix_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
+                                                           const float* __restrict__ x, float* __restrict__ y) {
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    const uint32_t d = code[i];
+    if (d == 0u) {
+        y[i] = 0.0f;
+        return;
+    }
+    int xx, yy, zz;
+    decode_cell(g, i, xx, yy, zz);
+    const double s = (double)d * x[i] -
+                     neighbour_sum<double>(g, per, xx, yy, zz, i, [&](uint32_t j) { return code[j] ? (double)x[j] : 0.0; });
+    y[i] = (float)s;
+}
+
+uint32_t periodic_bits(const ds_context* c);
+
+// Launches the direction kernel of iteration k (k == 0 has no previous direction).
+void launch_direction(ds_context* c, uint32_t per, double tol2, uint32_t k, bool first) {
+    PcgWork& w = c->pcg;
+    float* p_new = w.p[k & 1u];
+    const float* p_old = w.p[(k + 1u) & 1u];
+    if (k == 0)
+        pcg_direction_kernel<false><<<c->pcg.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, w.tiling, c->code, w.r, p_old, p_new, w.partials,
+                                                                         w.hist, w.status, tol2, k, first ? 1u : 0u);
+    else
+        pcg_direction_kernel<true><<<c->pcg.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, w.tiling, c->code, w.r, p_old, p_new, w.partials,
+                                                                        w.hist, w.status, tol2, k, first ? 1u : 0u);
+}
+
+uint32_t periodic_bits(const ds_context* c) {
+    return (c->pressure_periodic[0] ? 1u : 0u) | (c->pressure_periodic[1] ? 2u : 0u) | (c->pressure_periodic[2] ? 4u : 0u);
+}
+
+void launch_setup(ds_context* c, int kind, const RhsIn& in, const float* x0h, const float* x0l, bool x_is_guess,
+                  float* xh, float* xl) {
+    const uint32_t per = periodic_bits(c);
+    PcgWork& w = c->pcg;
+    switch (kind) {
+        case kRhsGiven:
+            pcg_setup_kernel<kRhsGiven><<<c->pcg.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, c->code, c->mask, in, c->fixed_term, w.b,
+                                                                             x0h, x0l, x_is_guess, xh, xl, w.r, w.partials, w.status);
+            break;
+        case kRhsForward:
+            pcg_setup_kernel<kRhsForward><<<c->pcg.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, c->code, c->mask, in, c->fixed_term,
+                                                                               w.b, x0h, x0l, x_is_guess, xh, xl, w.r, w.partials, w.status);
+            break;
+        default:
+            pcg_setup_kernel<kRhsAdjoint><<<c->pcg.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, c->code, c->mask, in, c->fixed_term,
+                                                                               w.b, x0h, x0l, x_is_guess, xh, xl, w.r, w.partials, w.status);
+            break;
+    }
+    check_launch("pcg_setup_kernel");
+}
+
+PcgStatus read_status(ds_context* c) {
+    DS_HIP(hipMemcpyAsync(&c->pcg.host_status[0], c->pcg.status, sizeof(PcgStatus), hipMemcpyDeviceToHost, c->stream));
+    DS_HIP(hipStreamSynchronize(c->stream));
+    return c->pcg.host_status[0];
+}
+
+// Iterates from the state left by launch_setup until the true residual meets tol or max_it.
+ds_solve_info pcg_iterate(ds_context* c, float* xh, float* xl, double tol, uint32_t max_it, bool warm, bool rejected) {
+    PcgWork& w = c->pcg;
+    const uint32_t per = periodic_bits(c);
+    const double tol2 = tol * tol;
+    const uint32_t R = c->replacement_interval;
+    ds_solve_info info{};
+    info.warm_started = warm ? 1u : 0u;
+    info.guess_rejected = rejected ? 1u : 0u;
+    uint32_t k = 0;
+    bool first = true;
+    uint32_t restarts = 0;
+    PcgStatus st{};
+    while (true) {
+        uint32_t batch = 1;
+        int slot = 0, pending = -1;
+        bool stop_seen = false;
+        while (k < max_it && !stop_seen) {
+            const uint32_t nb = std::min(batch, max_it - k);
+            for (uint32_t j = 0; j < nb; ++j, ++k) {
+                Profiler& prof = c->profiler;
+                hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr;
+                if (prof.every && k % prof.every == 0 && prof.used + 3 <= 6000) {
+                    e0 = prof.take();
+                    e1 = prof.take();
+                    e2 = prof.take();
+                }
+                if (e0) DS_HIP(hipEventRecord(e0, c->stream));
+                launch_direction(c, per, tol2, k, first);
+                first = false;
+                if (e1) DS_HIP(hipEventRecord(e1, c->stream));
+                pcg_update_kernel<<<c->pcg.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, w.tiling, c->code, w.p[k & 1u], w.r, xh, xl, w.partials,
+                                                                        w.hist, w.status, k);
+                if (e2) {
+                    DS_HIP(hipEventRecord(e2, c->stream));
+                    prof.recs.push_back({0, e0, e1});
+                    prof.recs.push_back({1, e1, e2});
+                }
+                if (prof.every) {
+                    prof.launches[0]++;
+                    prof.launches[1]++;
+                }
+                if ((k + 1u) % R == 0u && k + 1u < max_it)
+                    pcg_true_residual_kernel<<<c->pcg.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, c->code, w.b, xh, xl, w.r, w.partials,
+                                                                                   w.status, 1u);
+            }
+            check_launch("pcg iteration");
+            DS_HIP(hipMemcpyAsync(&w.host_status[slot], w.status, sizeof(PcgStatus), hipMemcpyDeviceToHost, c->stream));
+            DS_HIP(hipEventRecord(w.batch_event[slot], c->stream));
+            if (pending >= 0) {
+                DS_HIP(hipEventSynchronize(w.batch_event[pending]));
+                if (w.host_status[pending].stop) stop_seen = true;
+            }
+            pending = slot;
+            slot ^= 1;
+            batch = std::min<uint32_t>(batch * 2u, 32u);
+        }
+        if (!stop_seen && k >= max_it) {
+            // one more convergence test on the last update's residual
+            launch_direction(c, per, tol2, k, first);
+            first = false;
+            check_launch("pcg_direction_kernel");
+        }
+        st = read_status(c);
+        const uint32_t done = st.stop ? st.iterations : max_it;
+        pcg_true_residual_kernel<<<c->pcg.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, c->code, w.b, xh, xl, w.r, w.partials,
+                                                                       w.status, 0u);
+        pcg_finalize_kernel<<<1, kBlock, 0, c->stream>>>(w.partials, c->pcg.blocks, w.status, 0u);
+        check_launch("pcg_true_residual_kernel");
+        const PcgStatus fin = read_status(c);
+        info.iterations = done;
+        info.rhs_norm = std::sqrt(fin.bb);
+        info.final_residual = std::sqrt(std::max(0.0, fin.rr));
+        const bool finite = std::isfinite(fin.rr);
+        const bool ok = finite && fin.rr <= tol2 * fin.bb;
+        info.converged = ok ? 1u : 0u;
+        if (!finite) info.final_residual = fin.rr;
+        if (ok || !finite || !st.stop || done >= max_it || ++restarts > 8) break;
+        // the recursive residual met the tolerance but the true one does not: continue from the
+        // true residual (residual replacement) with the same search direction
+        pcg_clear_stop_kernel<<<1, 64, 0, c->stream>>>(w.status);
+        check_launch("pcg_clear_stop_kernel");
+        k = done;
+        first = false;
+    }
+    return info;
+}
+
+}  // namespace
+
+uint32_t pcg_periodic_bits(const ds_context* c) { return periodic_bits(c); }
+
+// Solves A x = b for the given right-hand-side kind. Returns the solve info; x = (xh, xl).
+ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], const float* const extra[3], float* xh,
+                             float* xl, const float* x0h, const float* x0l, double tol, uint32_t max_it) {
+    RhsIn in{};
+    for (int a = 0; a < 3; ++a) {
+        in.u[a] = u ? u[a] : nullptr;
+        in.extra[a] = extra ? extra[a] : nullptr;
+    }
+    const bool warm = x0h != nullptr;
+    bool rejected = false;
+    max_it = std::min(max_it, c->max_iterations);   // the per-iteration history is sized for this cap
+    launch_setup(c, kind, in, x0h, x0l, x0h == xh, xh, xl);
+    double rr0 = 0.0, bb = 0.0;
+    if (warm) {
+        // reject a non-finite guess or one whose residual exceeds ||b|| (worse than x0 = 0)
+        pcg_finalize_kernel<<<1, kBlock, 0, c->stream>>>(c->pcg.partials, c->pcg.blocks, c->pcg.status, 1u);
+        check_launch("pcg_finalize_kernel");
+        const PcgStatus s = read_status(c);
+        rr0 = s.rr;
+        bb = s.bb;
+        if (!(s.rr <= s.bb)) {
+            rejected = true;
+            launch_setup(c, kRhsGiven, in, nullptr, nullptr, false, xh, xl);
+        }
+    }
+    ds_solve_info info = pcg_iterate(c, xh, xl, tol, max_it, warm, rejected);
+    info.initial_residual = warm && !rejected ? std::sqrt(std::max(0.0, rr0)) : info.rhs_norm;
+    (void)bb;
+    return info;
+}
+
+ds_solve_info pcg_solve(ds_context* c, float* x_hi, float* x_lo, const float* x0, double tol, uint32_t max_it) {
+    return pcg_solve_kind(c, kRhsGiven, nullptr, nullptr, x_hi, x_lo, x0, nullptr, tol, max_it);
+}
+
+void launch_apply_matrix(ds_context* c, const float* x, float* y) {
+    apply_matrix_kernel<<<xcd_blocks_for(c->geo.n), kBlock, 0, c->stream>>>(c->geo, periodic_bits(c), c->code, x, y);
+    check_launch("apply_matrix_kernel");
+}
+
+}  // namespace ds
+
+using namespace ds;
+
+extern "C" {
+
+ds_status ds_pcg_solve(ds_context* c, const float* b, float* x, const float* x0, double tolerance, uint32_t max_iterations,
+                       ds_solve_info* info) {
+    if (!c || !b || !x) {
+        set_error("ds_pcg_solve: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    ds_solve_info si{};
+    ds_status st = guard([&] {
+        DS_HIP(hipMemcpyAsync(c->pcg.b, b, (size_t)c->geo.n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        const double tol = tolerance > 0.0 ? tolerance : c->tolerance;
+        const uint32_t mi = max_iterations ? max_iterations : c->max_iterations;
+        // solve into the context's pressure pair, then round hi + lo into x
+        si = pcg_solve_kind(c, kRhsGiven, nullptr, nullptr, c->pressure_hi, c->pressure_lo, x0, nullptr, tol, mi);
+        c->pressure_valid = false;
+    });
+    if (st != DS_OK) return st;
+    st = guard([&] {
+        combine_pair(c, c->pressure_hi, c->pressure_lo, x);
+        DS_HIP(hipStreamSynchronize(c->stream));
+    });
+    if (info) *info = si;
+    if (st != DS_OK) return st;
+    if (!si.converged) {
+        set_error(std::isfinite(si.final_residual) ? "ds_pcg_solve: not converged" : "ds_pcg_solve: non-finite residual");
+        return std::isfinite(si.final_residual) ? DS_ERROR_NOT_CONVERGED : DS_ERROR_NON_FINITE;
+    }
+    return DS_OK;
+}
+
+ds_status ds_profile_enable(ds_context* c, uint32_t sample_every) {
+    if (!c) {
+        set_error("ds_profile_enable: null context");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    c->profiler.every = sample_every;
+    c->profiler.launches[0] = c->profiler.launches[1] = 0;
+    return DS_OK;
+}
+
+ds_status ds_profile_read(ds_context* c, ds_kernel_timing* out, uint32_t capacity, uint32_t* count) {
+    if (!c || (capacity && !out)) {
+        set_error("ds_profile_read: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] {
+        DS_HIP(hipStreamSynchronize(c->stream));
+        Profiler& prof = c->profiler;
+        double total[2]{0.0, 0.0};
+        uint64_t samples[2]{0, 0};
+        for (const Profiler::Rec& r : prof.recs) {
+            float ms = 0.0f;
+            DS_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            total[r.kernel] += ms;
+            samples[r.kernel]++;
+        }
+        const char* names[2]{"pcg_direction_kernel", "pcg_update_kernel"};
+        const uint32_t n = std::min<uint32_t>(capacity, 2u);
+        for (uint32_t k = 0; k < n; ++k) {
+            std::snprintf(out[k].name, sizeof(out[k].name), "%s", names[k]);
+            out[k].launches = prof.launches[k];
+            out[k].samples = samples[k];
+            out[k].total_ms = total[k];
+        }
+        if (count) *count = n;
+        prof.recs.clear();
+        prof.used = 0;
+        prof.launches[0] = prof.launches[1] = 0;
+    });
+}
+
+ds_status ds_apply_pressure_matrix(ds_context* c, const float* x, float* y) {
+    if (!c || !x || !y) {
+        set_error("ds_apply_pressure_matrix: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] { launch_apply_matrix(c, x, y); });
+}
+
+}  // extern "C"

@@ -1,0 +1,405 @@
+"""Python host side of the MI355X smoke solver: a ctypes binding of libds_smoke.so.
+
+Mirrors the reference's Model interface (solvers/smoke/model.ixx:51-96: allocate_context,
+allocate_state, forward_step, vjp_step) and trajectory driver (solvers/solver.ixx:53-150) over the
+C ABI declared in include/ds/ds_smoke.h. PyTorch provides device memory and the stream; every
+computation runs in the HIP library, and loading fails loudly when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+from .scenes import cell_shape, face_shape
+
+_LIB = None
+
+
+class SolverError(RuntimeError):
+    def __init__(self, status: int, where: str, message: str):
+        super().__init__(f"{where}: {abi.STATUS_NAMES.get(status, status)} ({status}): {message}")
+        self.status = status
+
+
+def load_library(path: str | None = None):
+    """Load the in-tree HIP library (built by __graft_entry__.build())."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = path or abi.library_path()
+    if not os.path.exists(path):
+        raise RuntimeError(f"HIP smoke library not found at {path}; run __graft_entry__.build() first")
+    L = C.CDLL(path)
+    P = C.c_void_p
+    sigs = {
+        "ds_last_error": (C.c_char_p, []),
+        "ds_config_cell_count": (C.c_uint64, [C.POINTER(abi.Config)]),
+        "ds_config_face_count": (C.c_uint64, [C.POINTER(abi.Config), C.c_uint32]),
+        "ds_context_create": (C.c_int32, [C.POINTER(abi.Config), C.c_uint32, P, C.POINTER(P)]),
+        "ds_context_destroy": (C.c_int32, [P]),
+        "ds_context_stream": (P, [P]),
+        "ds_context_domain_info": (C.c_int32, [P, C.POINTER(abi.DomainInfo)]),
+        "ds_context_domain_masks": (C.c_int32, [P, P, P]),
+        "ds_allocate": (C.c_int32, [P, C.c_size_t, C.POINTER(P)]),
+        "ds_release": (C.c_int32, [P, P]),
+        "ds_copy_from_host": (C.c_int32, [P, P, P, C.c_size_t]),
+        "ds_copy_to_host": (C.c_int32, [P, P, P, C.c_size_t]),
+        "ds_copy_device": (C.c_int32, [P, P, P, C.c_size_t]),
+        "ds_zero": (C.c_int32, [P, P, C.c_size_t]),
+        "ds_synchronize": (C.c_int32, [P]),
+        "ds_source_forward": (C.c_int32, [P, P, P, P]),
+        "ds_source_vjp": (C.c_int32, [P, P, P, P]),
+        "ds_buoyancy_forward": (C.c_int32, [P, P, P, C.POINTER(P), C.POINTER(abi.Parameters), C.POINTER(P)]),
+        "ds_buoyancy_vjp": (C.c_int32, [P, P, P, C.POINTER(abi.Parameters), C.POINTER(P), P, P, C.POINTER(P),
+                                        C.POINTER(abi.ParameterAdjoint)]),
+        "ds_integrate_velocity_forward": (C.c_int32, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+        "ds_integrate_velocity_vjp": (C.c_int32, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+        "ds_advect_velocity_forward": (C.c_int32, [P, C.POINTER(P), C.POINTER(P)]),
+        "ds_advect_velocity_vjp": (C.c_int32, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+        "ds_constrain_velocity_forward": (C.c_int32, [P, C.POINTER(P), C.POINTER(P)]),
+        "ds_constrain_velocity_vjp": (C.c_int32, [P, C.POINTER(P), C.POINTER(P)]),
+        "ds_advect_scalar_forward": (C.c_int32, [P, C.c_uint32, P, C.POINTER(P), P]),
+        "ds_advect_scalar_vjp": (C.c_int32, [P, C.c_uint32, P, C.POINTER(P), P, P, C.POINTER(P)]),
+        "ds_projection_forward": (C.c_int32, [P, C.POINTER(P), P, P, P, C.POINTER(P), C.POINTER(abi.SolveInfo)]),
+        "ds_projection_vjp": (C.c_int32, [P, C.POINTER(P), C.POINTER(P), C.POINTER(abi.SolveInfo)]),
+        "ds_pcg_solve": (C.c_int32, [P, P, P, P, C.c_double, C.c_uint32, C.POINTER(abi.SolveInfo)]),
+        "ds_apply_pressure_matrix": (C.c_int32, [P, P, P]),
+        "ds_profile_enable": (C.c_int32, [P, C.c_uint32]),
+        "ds_profile_read": (C.c_int32, [P, C.POINTER(abi.KernelTiming), C.c_uint32, C.POINTER(C.c_uint32)]),
+        "ds_copy_state": (C.c_int32, [P, C.POINTER(abi.State), C.POINTER(abi.State)]),
+        "ds_copy_state_adjoint": (C.c_int32, [P, C.POINTER(abi.StateAdjoint), C.POINTER(abi.StateAdjoint)]),
+        "ds_accumulate_state_adjoint": (C.c_int32, [P, C.POINTER(abi.StateAdjoint), C.POINTER(abi.StateAdjoint)]),
+        "ds_forward_step": (C.c_int32, [P, C.POINTER(abi.State), C.POINTER(abi.Control), C.POINTER(abi.Parameters),
+                                        C.POINTER(abi.State), C.POINTER(abi.StepCache), C.POINTER(abi.StepInfo)]),
+        "ds_vjp_step": (C.c_int32, [P, C.POINTER(abi.State), C.POINTER(abi.Control), C.POINTER(abi.Parameters),
+                                    C.POINTER(abi.State), C.POINTER(abi.StepCache), C.POINTER(abi.StateAdjoint),
+                                    C.POINTER(abi.StateAdjoint), C.POINTER(abi.ControlAdjoint),
+                                    C.POINTER(abi.ParameterAdjoint), C.POINTER(abi.StepInfo)]),
+        "ds_simulate": (C.c_int32, [P, C.POINTER(abi.State), C.POINTER(abi.Control), C.c_uint32,
+                                    C.POINTER(abi.Parameters), C.c_uint32, C.POINTER(abi.State),
+                                    C.POINTER(abi.TrajectoryInfo)]),
+        "ds_loss_gradient": (C.c_int32, [P, C.POINTER(abi.State), C.POINTER(abi.Control), C.c_uint32,
+                                         C.POINTER(abi.Parameters), C.c_uint32, C.c_uint32, P, C.POINTER(abi.State),
+                                         C.POINTER(abi.StateAdjoint), C.POINTER(abi.ControlAdjoint),
+                                         C.POINTER(abi.ParameterAdjoint), C.POINTER(abi.TrajectoryInfo)]),
+    }
+    for name, (res, args) in sigs.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = L
+    return L
+
+
+def exported_symbols() -> list[str]:
+    """Entry points declared by include/ds/ds_smoke.h that this binding knows."""
+    load_library()
+    return sorted(n for n in dir(_LIB) if n.startswith("ds_"))
+
+
+def _ptr(t):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _p3(ts):
+    return (C.c_void_p * 3)(*[t.data_ptr() if t is not None else None for t in ts])
+
+
+class Context:
+    """An execution context (Model::allocate_context, model.cpp:32-182) on one GPU.
+
+    Fields are torch tensors in the reference layout: cell fields (nz, ny, nx), face component a
+    with extent n_a + 1 on its own axis. All work runs on this context's torch stream.
+    """
+
+    def __init__(self, config: abi.Config, mode: int = abi.MODE_DIFFERENTIABLE, device: int = 0):
+        import torch
+
+        self.torch = torch
+        self.lib = load_library()
+        self.config = config
+        self.device = torch.device("cuda", device)
+        torch.cuda.set_device(self.device)
+        self.stream = torch.cuda.Stream(device=self.device)
+        h = C.c_void_p()
+        st = self.lib.ds_context_create(C.byref(config), mode, C.c_void_p(self.stream.cuda_stream), C.byref(h))
+        self._check(st, "ds_context_create")
+        self.h = h
+        self.mode = mode
+        self.n = tuple(int(v) for v in config.resolution)
+        self.domain = abi.DomainInfo()
+        self._check(self.lib.ds_context_domain_info(self.h, C.byref(self.domain)), "ds_context_domain_info")
+
+    def _check(self, status, where):
+        if status != abi.DS_OK:
+            raise SolverError(status, where, self.lib.ds_last_error().decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ds_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- buffers
+    def _zeros(self, shape, dtype=None):
+        with self.torch.cuda.stream(self.stream):
+            return self.torch.zeros(shape, dtype=dtype or self.torch.float32, device=self.device)
+
+    def zeros_cell(self):
+        return self._zeros(cell_shape(self.n))
+
+    def zeros_faces(self):
+        return tuple(self._zeros(face_shape(self.n, a)) for a in range(3))
+
+    def to_device(self, a):
+        with self.torch.cuda.stream(self.stream):
+            return self.torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(self.device, non_blocking=False)
+
+    def synchronize(self):
+        self.stream.synchronize()
+
+    def allocate_state(self):
+        return {"density": self.zeros_cell(), "temperature": self.zeros_cell(), "velocity": self.zeros_faces()}
+
+    allocate_state_adjoint = allocate_state
+
+    def allocate_control(self):
+        return {"density_source": self.zeros_cell(), "temperature_source": self.zeros_cell(),
+                "external_acceleration": tuple(self.zeros_cell() for _ in range(3))}
+
+    allocate_control_adjoint = allocate_control
+
+    def allocate_parameters(self, values=(0.0, 0.0, 0.0, 0.0)):
+        return self.to_device(np.asarray(values, dtype=np.float32))
+
+    def allocate_parameter_adjoint(self):
+        return self._zeros((4,), dtype=self.torch.float64)
+
+    def allocate_step_cache(self, pressure=True):
+        return {"sourced_density": self.zeros_cell(), "sourced_temperature": self.zeros_cell(),
+                "force": tuple(self.zeros_cell() for _ in range(3)), "forced_velocity": self.zeros_faces(),
+                "advected_velocity": self.zeros_faces(),
+                "pressure": self._zeros((2,) + cell_shape(self.n)) if pressure else None, "pressure_guess": None}
+
+    def state_from_host(self, s):
+        return {"density": self.to_device(s["density"]), "temperature": self.to_device(s["temperature"]),
+                "velocity": tuple(self.to_device(v) for v in s["velocity"])}
+
+    def control_from_host(self, k):
+        return {"density_source": self.to_device(k["density_source"]),
+                "temperature_source": self.to_device(k["temperature_source"]),
+                "external_acceleration": tuple(self.to_device(v) for v in k["external_acceleration"])}
+
+    def to_host(self, d):
+        self.synchronize()
+        if isinstance(d, dict):
+            return {k: self.to_host(v) for k, v in d.items()}
+        if isinstance(d, (tuple, list)):
+            return tuple(self.to_host(v) for v in d)
+        if d is None:
+            return None
+        return d.detach().cpu().numpy()
+
+    # ---- C views
+    @staticmethod
+    def state_view(s):
+        return abi.State(s["density"].data_ptr(), s["temperature"].data_ptr(), (C.c_void_p * 3)(*[v.data_ptr() for v in s["velocity"]]))
+
+    @staticmethod
+    def control_view(k):
+        return abi.Control(k["density_source"].data_ptr(), k["temperature_source"].data_ptr(),
+                           (C.c_void_p * 3)(*[v.data_ptr() for v in k["external_acceleration"]]))
+
+    @staticmethod
+    def params_view(p):
+        base = p.data_ptr()
+        return abi.Parameters(base, base + 4, base + 8, base + 12)
+
+    @staticmethod
+    def param_adjoint_view(p):
+        base = p.data_ptr()
+        return abi.ParameterAdjoint(base, base + 8, base + 16, base + 24)
+
+    def cache_view(self, ch):
+        v = abi.StepCache()
+        v.sourced_density = ch["sourced_density"].data_ptr()
+        v.sourced_temperature = ch["sourced_temperature"].data_ptr()
+        for a in range(3):
+            v.force[a] = ch["force"][a].data_ptr() if ch.get("force") is not None else None
+            v.forced_velocity[a] = ch["forced_velocity"][a].data_ptr()
+            v.advected_velocity[a] = ch["advected_velocity"][a].data_ptr() if ch.get("advected_velocity") is not None else None
+        v.pressure = ch["pressure"].data_ptr() if ch.get("pressure") is not None else None
+        v.pressure_guess = ch["pressure_guess"].data_ptr() if ch.get("pressure_guess") is not None else None
+        return v
+
+    # ---- model steps (model.cpp:341-389)
+    def forward_step(self, state, control, params, next_state=None, cache=None, check=True):
+        next_state = next_state or self.allocate_state()
+        cache = cache or self.allocate_step_cache()
+        info = abi.StepInfo()
+        st = self.lib.ds_forward_step(self.h, C.byref(self.state_view(state)), C.byref(self.control_view(control)),
+                                      C.byref(self.params_view(params)), C.byref(self.state_view(next_state)),
+                                      C.byref(self.cache_view(cache)), C.byref(info))
+        if check:
+            self._check(st, "ds_forward_step")
+        return next_state, cache, info.projection
+
+    def vjp_step(self, state, control, params, next_state, cache, next_adj, prev_adj=None, control_adj=None,
+                 param_adj=None, check=True):
+        prev_adj = prev_adj or self.allocate_state_adjoint()
+        info = abi.StepInfo()
+        ca = C.byref(self.control_view(control_adj)) if control_adj is not None else None
+        pa = C.byref(self.param_adjoint_view(param_adj)) if param_adj is not None else None
+        st = self.lib.ds_vjp_step(self.h, C.byref(self.state_view(state)), C.byref(self.control_view(control)),
+                                  C.byref(self.params_view(params)), C.byref(self.state_view(next_state)),
+                                  C.byref(self.cache_view(cache)), C.byref(self.state_view(next_adj)),
+                                  C.byref(self.state_view(prev_adj)), ca, pa, C.byref(info))
+        if check:
+            self._check(st, "ds_vjp_step")
+        return prev_adj, info.projection
+
+    def accumulate_state_adjoint(self, src, dst):
+        self._check(self.lib.ds_accumulate_state_adjoint(self.h, C.byref(self.state_view(src)),
+                                                         C.byref(self.state_view(dst))), "ds_accumulate_state_adjoint")
+
+    def copy_state(self, src, dst):
+        self._check(self.lib.ds_copy_state(self.h, C.byref(self.state_view(src)), C.byref(self.state_view(dst))),
+                    "ds_copy_state")
+
+    # ---- trajectory (solver.ixx:53-150)
+    def simulate(self, initial, control, params, steps, final_state=None, check=True):
+        final_state = final_state or self.allocate_state()
+        info = abi.TrajectoryInfo()
+        st = self.lib.ds_simulate(self.h, C.byref(self.state_view(initial)), C.byref(self.control_view(control)), 1,
+                                  C.byref(self.params_view(params)), steps, C.byref(self.state_view(final_state)),
+                                  C.byref(info))
+        if check:
+            self._check(st, "ds_simulate")
+        return final_state, info
+
+    def loss_gradient(self, initial, control, params, steps, checkpoint_interval, target, initial_adj=None,
+                      control_adj=None, param_adj=None, final_state=None, check=True):
+        """0.5 ||rho_T - target||^2 and its gradient (checkpoint every k steps, recompute between)."""
+        initial_adj = initial_adj or self.allocate_state_adjoint()
+        info = abi.TrajectoryInfo()
+        ca = C.byref(self.control_view(control_adj)) if control_adj is not None else None
+        pa = C.byref(self.param_adjoint_view(param_adj)) if param_adj is not None else None
+        fs = C.byref(self.state_view(final_state)) if final_state is not None else None
+        st = self.lib.ds_loss_gradient(self.h, C.byref(self.state_view(initial)), C.byref(self.control_view(control)), 1,
+                                       C.byref(self.params_view(params)), steps, checkpoint_interval, _ptr(target), fs,
+                                       C.byref(self.state_view(initial_adj)), ca, pa, C.byref(info))
+        if check:
+            self._check(st, "ds_loss_gradient")
+        return info.loss, initial_adj, info
+
+    # ---- operators (tests)
+    def masks(self):
+        m = np.zeros(cell_shape(self.n), dtype=np.uint8)
+        c = np.zeros(cell_shape(self.n), dtype=np.uint8)
+        self._check(self.lib.ds_context_domain_masks(self.h, m.ctypes.data_as(C.c_void_p), c.ctypes.data_as(C.c_void_p)),
+                    "ds_context_domain_masks")
+        return m, c
+
+    def source_forward(self, s, src):
+        out = self.zeros_cell()
+        self._check(self.lib.ds_source_forward(self.h, _ptr(s), _ptr(src), _ptr(out)), "ds_source_forward")
+        return out
+
+    def source_vjp(self, oa, sa, srca):
+        self._check(self.lib.ds_source_vjp(self.h, _ptr(oa), _ptr(sa), _ptr(srca)), "ds_source_vjp")
+
+    def buoyancy_forward(self, rho, T, e, params):
+        f = tuple(self.zeros_cell() for _ in range(3))
+        self._check(self.lib.ds_buoyancy_forward(self.h, _ptr(rho), _ptr(T), _p3(e), C.byref(self.params_view(params)),
+                                                 _p3(f)), "ds_buoyancy_forward")
+        return f
+
+    def buoyancy_vjp(self, rho, T, params, fa, rho_adj, T_adj, e_adj, param_adj):
+        self._check(self.lib.ds_buoyancy_vjp(self.h, _ptr(rho), _ptr(T), C.byref(self.params_view(params)), _p3(fa),
+                                             _ptr(rho_adj), _ptr(T_adj), _p3(e_adj),
+                                             C.byref(self.param_adjoint_view(param_adj))), "ds_buoyancy_vjp")
+
+    def integrate_forward(self, u, f):
+        out = self.zeros_faces()
+        self._check(self.lib.ds_integrate_velocity_forward(self.h, _p3(u), _p3(f), _p3(out)), "ds_integrate_velocity_forward")
+        return out
+
+    def integrate_vjp(self, oa, ua, fa):
+        self._check(self.lib.ds_integrate_velocity_vjp(self.h, _p3(oa), _p3(ua), _p3(fa)), "ds_integrate_velocity_vjp")
+
+    def advect_velocity_forward(self, u):
+        out = self.zeros_faces()
+        self._check(self.lib.ds_advect_velocity_forward(self.h, _p3(u), _p3(out)), "ds_advect_velocity_forward")
+        return out
+
+    def advect_velocity_vjp(self, u, oa, ua):
+        self._check(self.lib.ds_advect_velocity_vjp(self.h, _p3(u), _p3(oa), _p3(ua)), "ds_advect_velocity_vjp")
+
+    def constrain_forward(self, u):
+        out = self.zeros_faces()
+        self._check(self.lib.ds_constrain_velocity_forward(self.h, _p3(u), _p3(out)), "ds_constrain_velocity_forward")
+        return out
+
+    def constrain_vjp(self, oa, ua):
+        self._check(self.lib.ds_constrain_velocity_vjp(self.h, _p3(oa), _p3(ua)), "ds_constrain_velocity_vjp")
+
+    def advect_scalar_forward(self, field, src, u):
+        out = self.zeros_cell()
+        self._check(self.lib.ds_advect_scalar_forward(self.h, field, _ptr(src), _p3(u), _ptr(out)), "ds_advect_scalar_forward")
+        return out
+
+    def advect_scalar_vjp(self, field, src, u, oa, src_adj, ua):
+        self._check(self.lib.ds_advect_scalar_vjp(self.h, field, _ptr(src), _p3(u), _ptr(oa), _ptr(src_adj), _p3(ua)),
+                    "ds_advect_scalar_vjp")
+
+    def projection_forward(self, u, guess=None, check=True):
+        out = self.zeros_faces()
+        p = self.zeros_cell()
+        rhs = self.zeros_cell()
+        info = abi.SolveInfo()
+        st = self.lib.ds_projection_forward(self.h, _p3(u), _ptr(p), _ptr(rhs), _ptr(guess), _p3(out), C.byref(info))
+        if check:
+            self._check(st, "ds_projection_forward")
+        return out, p, rhs, info
+
+    def projection_vjp(self, oa, ua=None, check=True):
+        ua = ua or self.zeros_faces()
+        info = abi.SolveInfo()
+        st = self.lib.ds_projection_vjp(self.h, _p3(oa), _p3(ua), C.byref(info))
+        if check:
+            self._check(st, "ds_projection_vjp")
+        return ua, info
+
+    def pcg_solve(self, b, x0=None, tolerance=0.0, max_iterations=0, check=True):
+        x = self.zeros_cell()
+        info = abi.SolveInfo()
+        st = self.lib.ds_pcg_solve(self.h, _ptr(b), _ptr(x), _ptr(x0), tolerance, max_iterations, C.byref(info))
+        if check:
+            self._check(st, "ds_pcg_solve")
+        return x, info, st
+
+    def profile_enable(self, sample_every: int):
+        """Bracket every `sample_every`-th launch of the PCG iteration kernels with HIP events."""
+        self._check(self.lib.ds_profile_enable(self.h, sample_every), "ds_profile_enable")
+
+    def profile_read(self):
+        """{kernel name: (launches, sampled launches, average ms per sampled launch)}."""
+        out = (abi.KernelTiming * 8)()
+        n = C.c_uint32()
+        self._check(self.lib.ds_profile_read(self.h, out, 8, C.byref(n)), "ds_profile_read")
+        return {out[i].name.decode(): (out[i].launches, out[i].samples,
+                                       out[i].total_ms / out[i].samples if out[i].samples else float("nan"))
+                for i in range(n.value)}
+
+    def apply_matrix(self, x):
+        y = self.zeros_cell()
+        self._check(self.lib.ds_apply_pressure_matrix(self.h, _ptr(x), _ptr(y)), "ds_apply_pressure_matrix")
+        return y

@@ -1,0 +1,332 @@
+/*
+ * ds_smoke.h — C ABI of the MI355X (gfx950) smoke solver: forward step, hand-written VJP,
+ * Jacobi-PCG pressure projection and its adjoint, checkpointed trajectory gradient.
+ *
+ * Every entry point takes plain pointers and sizes. Field pointers are DEVICE pointers in the
+ * reference's layout (cuda_types.h:7-129, operators.cu:36-47): one allocation per component,
+ * x-fastest indexing x + nx*(y + ny*z); axis-a face arrays have extent n_a + 1 on their own axis.
+ * Cell fields hold nx*ny*nz floats, face component a holds ds_face_count(a) floats.
+ *
+ * Element types follow the build's precision rule: primal fields, stencils and adjoint fields
+ * are FP32 (the reference's adjoints are FP64, data.ixx:101-115); CG dot products, residual
+ * norms and the four scalar parameter adjoints are FP64.
+ *
+ * Errors: every function returns a ds_status (0 = success). ds_last_error() returns a
+ * thread-local description of the last failure (replaces the std::runtime_error thrown by
+ * check_launch, operators.cu:28-30, and by xayah::cuda::Resource, cuda.cpp:13-66).
+ *
+ * Ownership (operators.h:40-72, model.cpp:185-339): the caller owns states, controls,
+ * parameters, step caches and adjoints. The context owns its scratch (PCG vectors, adjoint
+ * scratch, domain masks). No step function allocates. Adjoint outputs are ACCUMULATED (+=)
+ * into caller buffers, as in the reference (model.cpp:279-287, operators.cu:297-302).
+ * All work is stream-ordered on the context stream; one context per host thread.
+ */
+#ifndef DS_SMOKE_H
+#define DS_SMOKE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t ds_status;
+enum {
+    DS_OK = 0,
+    DS_ERROR_INVALID_ARGUMENT = 1,
+    DS_ERROR_OUT_OF_MEMORY = 2,
+    DS_ERROR_LAUNCH = 3,
+    DS_ERROR_NOT_CONVERGED = 4,
+    DS_ERROR_NON_FINITE = 5,
+    DS_ERROR_RUNTIME = 6,
+    DS_ERROR_UNSUPPORTED = 7
+};
+
+/* Boundary modes, values as the reference enums (data.ixx:14-25). Face order for modes[] and
+ * values[]: x_min, x_max, y_min, y_max, z_min, z_max (operators.cpp:85-105). Velocity value of
+ * face f, component a is values[3*f + a]. */
+enum { DS_SCALAR_FIXED_VALUE = 0, DS_SCALAR_ZERO_GRADIENT = 1, DS_SCALAR_PERIODIC = 2 };
+enum {
+    DS_VELOCITY_FIXED_VALUE = 0,
+    DS_VELOCITY_ZERO_GRADIENT = 1,
+    DS_VELOCITY_NORMAL_FIXED_TANGENT_ZERO_GRADIENT = 2,
+    DS_VELOCITY_PERIODIC = 3
+};
+
+typedef struct ds_scalar_boundary {   /* cuda_types.h:21-24 */
+    uint32_t modes[6];
+    float values[6];
+} ds_scalar_boundary;
+
+typedef struct ds_velocity_boundary { /* cuda_types.h:26-29 */
+    uint32_t modes[6];
+    float values[18];
+} ds_velocity_boundary;
+
+/* Collider (data.ixx:55-70): shape 0 = ellipsoid (size = radii), 1 = box (size = half extents). */
+enum { DS_SHAPE_ELLIPSOID = 0, DS_SHAPE_BOX = 1 };
+typedef struct ds_collider {
+    uint32_t shape;
+    float center[3];
+    float size[3];
+    float velocity[3];
+    float density;
+    float temperature;
+} ds_collider;
+
+/* Pressure solver. PCG is the build's solver (Jacobi-preconditioned CG on the reference's
+ * matrix, operators.cu:852-902); RBGS is the reference's fixed-count red-black Gauss-Seidel
+ * (operators.cu:885-902,1004-1013), kept as a reference mode. */
+enum { DS_PROJECTION_PCG = 0, DS_PROJECTION_RBGS = 1 };
+
+/* Initial guess of the forward and adjoint pressure solves. */
+enum { DS_WARM_START_NONE = 0, DS_WARM_START_PREVIOUS = 1, DS_WARM_START_CNN = 2 };
+
+/* smoke::Configuration (data.ixx:72-83) plus the solver fields this build adds. */
+typedef struct ds_config {
+    uint32_t resolution[3];
+    float cell_size;
+    float time_step;
+    uint32_t pressure_iterations;           /* RBGS sweeps (reference mode only) */
+    ds_velocity_boundary velocity_boundary;
+    ds_scalar_boundary pressure_boundary;
+    ds_scalar_boundary density_boundary;
+    ds_scalar_boundary temperature_boundary;
+    const ds_collider* colliders;           /* host array, read at context creation only */
+    uint32_t collider_count;
+    uint32_t vorticity_confinement_enabled; /* must be 0: vorticity confinement is not built */
+    uint32_t projection_solver;             /* DS_PROJECTION_* */
+    uint32_t cg_max_iterations;             /* 0 selects 4 * (nx + ny + nz) + 1000 */
+    double cg_tolerance;                    /* ||b - A x||_2 <= tol * ||b||_2, FP64 norms; 0 selects 1e-6 */
+    uint32_t warm_start;                    /* DS_WARM_START_* */
+    uint32_t residual_replacement_interval; /* true-residual refresh period, 0 selects 50 */
+} ds_config;
+
+/* ExecutionMode (model.ixx:10-13). */
+enum { DS_MODE_FORWARD = 0, DS_MODE_DIFFERENTIABLE = 1 };
+
+/* Field views: device pointers (data.ixx:117-134, 155-196). */
+typedef struct ds_state {
+    float* density;
+    float* temperature;
+    float* velocity[3];
+} ds_state;
+
+typedef struct ds_control {
+    float* density_source;
+    float* temperature_source;
+    float* external_acceleration[3];
+} ds_control;
+
+typedef struct ds_parameters {          /* each a device pointer to ONE float */
+    float* ambient_temperature;
+    float* density_buoyancy;
+    float* temperature_buoyancy;
+    float* vorticity_confinement;
+} ds_parameters;
+
+/* StepCache (data.ixx:189-196) without the vorticity cache. force and advected_velocity may be
+ * NULL (context scratch is used; the VJP does not read them). The pressure is kept as a
+ * compensated FP32 pair (hi then lo, 2 * cell_count floats): pressure (may be NULL) receives
+ * the converged pair of the forward step; pressure_guess (may be NULL) is read as the initial
+ * guess of its pressure solve, e.g. the pair a first pass over the same step stored. */
+typedef struct ds_step_cache {
+    float* sourced_density;
+    float* sourced_temperature;
+    float* force[3];
+    float* forced_velocity[3];
+    float* advected_velocity[3];
+    float* pressure;
+    const float* pressure_guess;
+} ds_step_cache;
+
+typedef struct ds_state_adjoint {
+    float* density;
+    float* temperature;
+    float* velocity[3];
+} ds_state_adjoint;
+
+typedef struct ds_control_adjoint {
+    float* density_source;
+    float* temperature_source;
+    float* external_acceleration[3];
+} ds_control_adjoint;
+
+typedef struct ds_parameter_adjoint {   /* each a device pointer to ONE double */
+    double* ambient_temperature;
+    double* density_buoyancy;
+    double* temperature_buoyancy;
+    double* vorticity_confinement;
+} ds_parameter_adjoint;
+
+typedef struct ds_solve_info {
+    uint32_t iterations;
+    uint32_t converged;
+    uint32_t warm_started;              /* 1 when a non-zero initial guess was used */
+    uint32_t guess_rejected;            /* 1 when the guess was non-finite or worse than x0 = 0 */
+    double rhs_norm;                    /* ||b||_2 */
+    double initial_residual;            /* ||b - A x0||_2 */
+    double final_residual;              /* true residual at exit */
+} ds_solve_info;
+
+typedef struct ds_step_info {
+    ds_solve_info projection;
+} ds_step_info;
+
+typedef struct ds_trajectory_info {
+    uint32_t steps;
+    uint32_t checkpoint_interval;
+    uint32_t forward_solves, recompute_solves, adjoint_solves;
+    uint32_t not_converged;
+    uint64_t forward_iterations, recompute_iterations, adjoint_iterations;
+    uint32_t max_iterations;
+    uint32_t guesses_rejected;
+    double max_relative_residual;
+    double loss;
+} ds_trajectory_info;
+
+typedef struct ds_domain_info {
+    uint64_t cell_count;
+    uint64_t face_count[3];
+    uint64_t unknown_count;             /* PCG unknowns: fluid cells minus anchors */
+    uint64_t solid_count;
+    uint64_t pressure_anchor;           /* reference anchor (model.cpp:72-77); cell_count if none */
+    uint32_t extra_anchor_count;        /* anchors of fluid pockets cut off from the main anchor */
+    uint32_t has_fixed_pressure;
+} ds_domain_info;
+
+typedef struct ds_context ds_context;
+
+/* ---- error reporting ------------------------------------------------------------------- */
+const char* ds_last_error(void);
+/* Number of cells / faces of a configuration without a context (sizes for the caller's buffers). */
+uint64_t ds_config_cell_count(const ds_config* config);
+uint64_t ds_config_face_count(const ds_config* config, uint32_t axis);
+
+/* ---- context and device resource (cuda.ixx:7-28, model.cpp:32-182) --------------------- */
+/* Model::allocate_context: validates the configuration, rasterises colliders, picks the
+ * pressure anchor, builds the per-cell connectivity code and allocates scratch. stream may be
+ * NULL (the context creates its own non-blocking stream) or a hipStream_t to run on. */
+ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream, ds_context** out);
+ds_status ds_context_destroy(ds_context* context);
+void* ds_context_stream(ds_context* context);
+ds_status ds_context_domain_info(ds_context* context, ds_domain_info* info);
+/* Copies the u8 solid mask and the u8 connectivity code (diagonal count, 0 = not an unknown)
+ * to host arrays of cell_count bytes. Either pointer may be NULL. Synchronises. */
+ds_status ds_context_domain_masks(ds_context* context, uint8_t* mask, uint8_t* code);
+/* Resource::allocate / release / copy_* / zero / synchronize (cuda.cpp:39-66). */
+ds_status ds_allocate(ds_context* context, size_t bytes, void** out);
+ds_status ds_release(ds_context* context, void* pointer);
+ds_status ds_copy_from_host(ds_context* context, void* device, const void* host, size_t bytes);
+ds_status ds_copy_to_host(ds_context* context, void* host, const void* device, size_t bytes);
+ds_status ds_copy_device(ds_context* context, void* destination, const void* source, size_t bytes);
+ds_status ds_zero(ds_context* context, void* device, size_t bytes);
+ds_status ds_synchronize(ds_context* context);
+
+/* ---- operators (cuda_kernels launchers, operators.h:40-72) ----------------------------- */
+/* source_forward (operators.cu:286-290): out = mask ? 0 : state + dt * source. */
+ds_status ds_source_forward(ds_context* c, const float* state, const float* source, float* out);
+/* source_vjp (operators.cu:297-302): state_adj += out_adj, source_adj += dt * out_adj (fluid). */
+ds_status ds_source_vjp(ds_context* c, const float* out_adj, float* state_adj, float* source_adj);
+/* buoyancy_forward (operators.cu:304-313). */
+ds_status ds_buoyancy_forward(ds_context* c, const float* density, const float* temperature,
+                              float* const external_acceleration[3], const ds_parameters* params,
+                              float* force[3]);
+/* buoyancy_vjp (operators.cu:326-337); parameter adjoints are FP64 dot products. */
+ds_status ds_buoyancy_vjp(ds_context* c, const float* density, const float* temperature,
+                          const ds_parameters* params, float* const force_adj[3], float* density_adj,
+                          float* temperature_adj, float* external_acceleration_adj[3],
+                          const ds_parameter_adjoint* param_adj);
+/* integrate_velocity_forward (operators.cu:339-364). */
+ds_status ds_integrate_velocity_forward(ds_context* c, float* const velocity[3], float* const force[3],
+                                        float* out[3]);
+/* integrate_velocity_vjp (operators.cu:366-379). */
+ds_status ds_integrate_velocity_vjp(ds_context* c, float* const out_adj[3], float* velocity_adj[3],
+                                    float* force_adj[3]);
+/* advect_velocity_forward (operators.cu:381-388): raw self-advection, no constraint. */
+ds_status ds_advect_velocity_forward(ds_context* c, float* const velocity[3], float* out[3]);
+/* advect_velocity_vjp (operators.cu:415-439). */
+ds_status ds_advect_velocity_vjp(ds_context* c, float* const velocity[3], float* const out_adj[3],
+                                 float* velocity_adj[3]);
+/* constrain_velocity_forward (operators.cu:466-481). */
+ds_status ds_constrain_velocity_forward(ds_context* c, float* const velocity[3], float* out[3]);
+/* constrain_velocity_vjp (operators.cu:483-497). */
+ds_status ds_constrain_velocity_vjp(ds_context* c, float* const out_adj[3], float* velocity_adj[3]);
+/* advect_scalar_forward (operators.cu:499-507); field 0 = density, 1 = temperature selects the
+ * boundary (operators.cpp:170-172) and the collider value. */
+ds_status ds_advect_scalar_forward(ds_context* c, uint32_t field, const float* source,
+                                   float* const velocity[3], float* out);
+/* advect_scalar_vjp (operators.cu:537-561). */
+ds_status ds_advect_scalar_vjp(ds_context* c, uint32_t field, const float* source,
+                               float* const velocity[3], const float* out_adj, float* source_adj,
+                               float* velocity_adj[3]);
+/* projection_forward (operators.cu:1004-1013) with the pressure solve replaced by Jacobi-PCG to
+ * the configured relative residual. pressure and rhs are cell fields (rhs receives b of A p = b).
+ * guess may be NULL. info may be NULL. */
+ds_status ds_projection_forward(ds_context* c, float* const velocity[3], float* pressure, float* rhs,
+                                const float* guess, float* out[3], ds_solve_info* info);
+/* projection_vjp (operators.cu:1020-1031) with the unrolled RBGS reverse replaced by the adjoint
+ * PCG solve A lambda = p_bar on the same SPD matrix. velocity_adj += (transpose of projection). */
+ds_status ds_projection_vjp(ds_context* c, float* const out_adj[3], float* velocity_adj[3],
+                            ds_solve_info* info);
+/* Jacobi-PCG on the projection matrix A (operators.cu:852-902): solves A x = b. x0 may be NULL. */
+ds_status ds_pcg_solve(ds_context* c, const float* b, float* x, const float* x0, double tolerance,
+                       uint32_t max_iterations, ds_solve_info* info);
+/* y = A x on unknowns (0 elsewhere); for tests. */
+ds_status ds_apply_pressure_matrix(ds_context* c, const float* x, float* y);
+
+/* ---- kernel timing (no counterpart in the reference, which times whole steps with
+ * std::chrono, examples/smoke/forward/simulation.cpp:91,107-119) -------------------------- */
+/* Brackets every `sample_every`-th launch of the two PCG iteration kernels with HIP events on
+ * the context stream (0 disables). */
+typedef struct ds_kernel_timing {
+    char name[48];
+    uint64_t launches;   /* launches since profiling was enabled */
+    uint64_t samples;    /* launches bracketed by events */
+    double total_ms;     /* summed event time of the sampled launches */
+} ds_kernel_timing;
+ds_status ds_profile_enable(ds_context* c, uint32_t sample_every);
+/* Synchronises, writes up to `capacity` entries and their number, and clears the samples. */
+ds_status ds_profile_read(ds_context* c, ds_kernel_timing* out, uint32_t capacity, uint32_t* count);
+
+/* ---- model steps (model.ixx:57-92, model.cpp:332-389) ---------------------------------- */
+ds_status ds_copy_state(ds_context* c, const ds_state* source, ds_state* destination);
+ds_status ds_copy_state_adjoint(ds_context* c, const ds_state_adjoint* source, ds_state_adjoint* destination);
+ds_status ds_accumulate_state_adjoint(ds_context* c, const ds_state_adjoint* source, ds_state_adjoint* destination);
+/* Model::forward_step (model.cpp:341-349). info may be NULL. */
+ds_status ds_forward_step(ds_context* c, const ds_state* state, const ds_control* control,
+                          const ds_parameters* params, ds_state* next_state, ds_step_cache* cache,
+                          ds_step_info* info);
+/* Model::vjp_step (model.cpp:361-389). Reads next_state->velocity and the cache's sourced fields
+ * and forced velocity. prev_adj, control_adj and param_adj are accumulated (+=). */
+ds_status ds_vjp_step(ds_context* c, const ds_state* state, const ds_control* control,
+                      const ds_parameters* params, const ds_state* next_state,
+                      const ds_step_cache* cache, const ds_state_adjoint* next_adj,
+                      ds_state_adjoint* prev_adj, ds_control_adjoint* control_adj,
+                      const ds_parameter_adjoint* param_adj, ds_step_info* info);
+
+/* ---- trajectory driver (solver.ixx:53-81,115-150) --------------------------------------- */
+/* Runs `steps` forward steps from `initial` into `final_state`. controls holds control_count
+ * views: 1 (the same control every step) or `steps`. */
+ds_status ds_simulate(ds_context* c, const ds_state* initial, const ds_control* controls,
+                      uint32_t control_count, const ds_parameters* params, uint32_t steps,
+                      ds_state* final_state, ds_trajectory_info* info);
+/* Loss 0.5 * ||density_T - target||^2 after `steps` steps and its gradient by a reverse sweep
+ * with a checkpoint every `checkpoint_interval` steps and recompute in between (peak memory
+ * (steps/k + k) states). initial_adj receives dL/d(initial state) (+=); control_adj (may be
+ * NULL) receives the sum over steps of dL/d(control) when control_count == 1, else it points
+ * to `steps` views; param_adj (may be NULL) accumulates dL/d(parameters). final_state may be
+ * NULL. */
+ds_status ds_loss_gradient(ds_context* c, const ds_state* initial, const ds_control* controls,
+                           uint32_t control_count, const ds_parameters* params, uint32_t steps,
+                           uint32_t checkpoint_interval, const float* target_density,
+                           ds_state* final_state, ds_state_adjoint* initial_adj,
+                           ds_control_adjoint* control_adj, const ds_parameter_adjoint* param_adj,
+                           ds_trajectory_info* info);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DS_SMOKE_H */

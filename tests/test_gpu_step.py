@@ -1,0 +1,132 @@
+"""Model-step and trajectory parity of the HIP path against the FP64 oracle.
+
+forward_step / vjp_step (solvers/smoke/model.cpp:341-389) on three scenes, a multi-step plume
+trajectory, and the checkpointed loss gradient (solvers/solver.ixx:115-150 with a checkpoint
+every k steps) against the oracle's gradient of the same discretisation.
+"""
+import numpy as np
+import pytest
+
+from helpers import SCENES, random_fields, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _inputs(sc, name):
+    from ds_smoke import scenes
+
+    if name == "closed":
+        return scenes.initial_state(sc), scenes.control(sc)
+    f = random_fields(sc, seed=3, vel_scale=0.4)
+    state = {"density": f["rho"], "temperature": f["T"], "velocity": f["vel"]}
+    control = {"density_source": f["s_rho"], "temperature_source": f["s_T"], "external_acceleration": f["e"]}
+    return state, control
+
+
+@pytest.fixture(scope="module", params=sorted(SCENES))
+def step_setup(request):
+    import oracle as O
+    from ds_smoke import abi
+    from ds_smoke.solver import Context
+
+    sc = SCENES[request.param]()
+    ctx = Context(sc.config(tolerance=1e-6), abi.MODE_DIFFERENTIABLE)
+    orc = O.Oracle(sc, tolerance=1e-10)
+    return request.param, sc, ctx, orc
+
+
+def test_forward_step(step_setup):
+    name, sc, ctx, orc = step_setup
+    s, k = _inputs(sc, name)
+    params = ctx.allocate_parameters(sc.params)
+    nxt, cache, info = ctx.forward_step(ctx.state_from_host(s), ctx.control_from_host(k), params)
+    assert info.converged
+    ref, ref_cache, _ = orc.forward_step(s, k, sc.params)
+    got = ctx.to_host(nxt)
+    assert rel(got["density"], ref["density"]) < 1e-5, rel(got["density"], ref["density"])
+    assert rel(got["temperature"], ref["temperature"]) < 1e-5
+    for a in range(3):
+        assert rel(got["velocity"][a], ref["velocity"][a]) < 1e-5, (name, a, rel(got["velocity"][a], ref["velocity"][a]))
+    gc = ctx.to_host(cache)
+    for a in range(3):
+        assert rel(gc["forced_velocity"][a], ref_cache["forced_velocity"][a]) < 1e-6
+        assert rel(gc["advected_velocity"][a], ref_cache["advected_velocity"][a]) < 5e-5
+    p = gc["pressure"][0].astype(np.float64) + gc["pressure"][1]
+    assert rel(p, ref_cache["pressure"]) < 1e-4
+
+
+def test_vjp_step(step_setup):
+    name, sc, ctx, orc = step_setup
+    s, k = _inputs(sc, name)
+    params = ctx.allocate_parameters(sc.params)
+    st, kt = ctx.state_from_host(s), ctx.control_from_host(k)
+    nxt, cache, _ = ctx.forward_step(st, kt, params)
+    ref, ref_cache, _ = orc.forward_step(s, k, sc.params)
+    rng = np.random.default_rng(11)
+    adj = {"density": rng.standard_normal(s["density"].shape).astype(np.float32),
+           "temperature": rng.standard_normal(s["density"].shape).astype(np.float32),
+           "velocity": tuple(rng.standard_normal(v.shape).astype(np.float32) for v in s["velocity"])}
+    ca = ctx.allocate_control_adjoint()
+    pa = ctx.allocate_parameter_adjoint()
+    prev, info = ctx.vjp_step(st, kt, params, nxt, cache, ctx.state_from_host(adj), control_adj=ca, param_adj=pa)
+    assert info.converged
+    oca = {"density_source": orc.zeros_cell(), "temperature_source": orc.zeros_cell(),
+           "external_acceleration": tuple(orc.zeros_cell() for _ in range(3))}
+    opa = np.zeros(4)
+    ref_prev, _ = orc.vjp_step(ref, ref_cache, sc.params, adj, control_adj=oca, param_adj=opa)
+    g = ctx.to_host(prev)
+    for key in ("density", "temperature"):
+        assert rel(g[key], ref_prev[key]) < 1e-4, (name, key, rel(g[key], ref_prev[key]))
+    for a in range(3):
+        assert rel(g["velocity"][a], ref_prev["velocity"][a]) < 1e-4, (name, a, rel(g["velocity"][a], ref_prev["velocity"][a]))
+    gca = ctx.to_host(ca)
+    assert rel(gca["density_source"], oca["density_source"]) < 1e-4
+    for a in range(3):
+        assert rel(gca["external_acceleration"][a], oca["external_acceleration"][a]) < 1e-4
+    # the parameter adjoints are sums with heavy cancellation (random adjoint inputs), so the
+    # FP32 terms are compared with an absolute floor at their own scale
+    np.testing.assert_allclose(ctx.to_host(pa)[:3], opa[:3], rtol=1e-4, atol=1e-4)
+
+
+def test_plume_trajectory_16():
+    """20 plume steps at 16^3: density within 1e-5 relative L2 of the oracle."""
+    import oracle as O
+    from ds_smoke import abi, scenes
+    from ds_smoke.solver import Context
+
+    sc = scenes.plume(16)
+    ctx = Context(sc.config(tolerance=1e-6), abi.MODE_FORWARD)
+    orc = O.Oracle(sc, tolerance=1e-10)
+    s0, k = scenes.initial_state(sc), scenes.control(sc)
+    params = ctx.allocate_parameters(sc.params)
+    fin, info = ctx.simulate(ctx.state_from_host(s0), ctx.control_from_host(k), params, 20)
+    assert info.forward_solves == 20 and info.not_converged == 0
+    ref, _ = orc.simulate(s0, k, sc.params, 20)
+    got = ctx.to_host(fin)
+    assert rel(got["density"], ref["density"]) < 1e-5, rel(got["density"], ref["density"])
+    assert rel(got["temperature"], ref["temperature"]) < 1e-5
+
+
+@pytest.mark.parametrize("k", [1, 3, 8])
+def test_loss_gradient_matches_oracle(k):
+    """Checkpointed gradient of 0.5||rho_T - t||^2 w.r.t. the initial state, 8 steps at 16^3."""
+    import oracle as O
+    from ds_smoke import abi, scenes
+    from ds_smoke.solver import Context
+
+    sc = scenes.plume(16)
+    ctx = Context(sc.config(tolerance=1e-6), abi.MODE_DIFFERENTIABLE)
+    orc = O.Oracle(sc, tolerance=1e-10)
+    s0, kk = scenes.initial_state(sc), scenes.control(sc)
+    params = ctx.allocate_parameters(sc.params)
+    pa = ctx.allocate_parameter_adjoint()
+    loss, grad, info = ctx.loss_gradient(ctx.state_from_host(s0), ctx.control_from_host(kk), params, 8, k,
+                                         ctx.to_device(sc.target), param_adj=pa)
+    assert info.forward_solves == 8 and info.recompute_solves == 8 and info.adjoint_solves == 8
+    assert info.not_converged == 0
+    ref = orc.loss_gradient(s0, kk, sc.params, 8, k, sc.target)
+    assert abs(loss - ref["loss"]) / ref["loss"] < 1e-5
+    g = ctx.to_host(grad)
+    for a in range(3):
+        assert rel(g["velocity"][a], ref["grad"]["velocity"][a]) < 2e-3, (a, rel(g["velocity"][a], ref["grad"]["velocity"][a]))
+    np.testing.assert_allclose(ctx.to_host(pa)[:3], ref["params"][:3], rtol=2e-3)
