@@ -82,6 +82,8 @@ def main():
     ap.add_argument("--profile-every", type=int, default=8)
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--warm-start", choices=["none", "previous"], default="none",
+                    help="initial guess of each pressure solve: zero, or the previous solve's solution")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the hot path is single-GPU here); got --gpus {args.gpus}", file=sys.stderr)
@@ -92,7 +94,8 @@ def main():
     from ds_smoke.solver import Context
 
     sc = scenes.plume(args.grid)
-    ctx = Context(sc.config(tolerance=1e-6), abi.MODE_DIFFERENTIABLE)
+    warm = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS}[args.warm_start]
+    ctx = Context(sc.config(tolerance=1e-6, warm_start=warm), abi.MODE_DIFFERENTIABLE)
     s0 = ctx.state_from_host(scenes.initial_state(sc))
     ctl = ctx.control_from_host(scenes.control(sc))
     params = ctx.allocate_parameters(sc.params)
@@ -155,9 +158,9 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plume{args.grid}_fwd{args.sim_steps}_vjp_checkpoint{args.checkpoint}",
                    "grid": args.grid, "sim_steps_per_bench_step": args.sim_steps,
-                   "checkpoint_interval": args.checkpoint, "cg_tolerance": 1e-6, "warm_start": "none",
+                   "checkpoint_interval": args.checkpoint, "cg_tolerance": 1e-6, "warm_start": args.warm_start,
                    "pressure_tape": True},
-        "cg_iterations_per_solve": {"plain": {k: round(v, 1) for k, v in per_solve.items()}},
+        "cg_iterations_per_solve": {("plain" if args.warm_start == "none" else args.warm_start): {k: round(v, 1) for k, v in per_solve.items()}},
         "loss": loss,
         "roofline": roofline,
         "kernels": {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in kern.items()},
