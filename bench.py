@@ -24,8 +24,12 @@ HBM_PEAK_GBS = 8000.0
 # Algorithmic bytes per cell of one launch of each PCG iteration kernel: unknowns read/write
 # their FP32 vectors and the u8 code once; other cells only read the code (and the direction
 # kernel writes p = 0 there).
-BYTES_UNKNOWN = {"pcg_direction_kernel": 4 + 4 + 1 + 4, "pcg_update_kernel": 4 + 4 + 1 + 8 + 4 + 8}
-BYTES_OTHER = {"pcg_direction_kernel": 1 + 4, "pcg_update_kernel": 1}
+# pcg_fused_kernel (single-reduction form, one launch per iteration): reads code, r, w, s, p and
+# the solution pair, writes r, w, s, p and the pair on unknowns; other cells read the code and
+# write zeros into the next r and w.
+BYTES_UNKNOWN = {"pcg_direction_kernel": 4 + 4 + 1 + 4, "pcg_update_kernel": 4 + 4 + 1 + 8 + 4 + 8,
+                 "pcg_fused_kernel": 1 + 4 * 4 + 8 + 4 * 4 + 8}
+BYTES_OTHER = {"pcg_direction_kernel": 1 + 4, "pcg_update_kernel": 1, "pcg_fused_kernel": 1 + 4 + 4}
 
 
 def host_threads():
@@ -84,6 +88,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--warm-start", choices=["none", "previous"], default="none",
                     help="initial guess of each pressure solve: zero, or the previous solve's solution")
+    ap.add_argument("--cg-variant", choices=["single", "two"], default="single",
+                    help="PCG recurrence: single-reduction (one fused kernel per iteration) or two-reduction")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the hot path is single-GPU here); got --gpus {args.gpus}", file=sys.stderr)
@@ -95,7 +101,8 @@ def main():
 
     sc = scenes.plume(args.grid)
     warm = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS}[args.warm_start]
-    ctx = Context(sc.config(tolerance=1e-6, warm_start=warm), abi.MODE_DIFFERENTIABLE)
+    variant = {"single": abi.CG_SINGLE_REDUCTION, "two": abi.CG_TWO_REDUCTION}[args.cg_variant]
+    ctx = Context(sc.config(tolerance=1e-6, warm_start=warm, cg_variant=variant), abi.MODE_DIFFERENTIABLE)
     s0 = ctx.state_from_host(scenes.initial_state(sc))
     ctl = ctx.control_from_host(scenes.control(sc))
     params = ctx.allocate_parameters(sc.params)
@@ -159,6 +166,7 @@ def main():
         "config": {"workload": f"plume{args.grid}_fwd{args.sim_steps}_vjp_checkpoint{args.checkpoint}",
                    "grid": args.grid, "sim_steps_per_bench_step": args.sim_steps,
                    "checkpoint_interval": args.checkpoint, "cg_tolerance": 1e-6, "warm_start": args.warm_start,
+                   "cg_variant": args.cg_variant,
                    "pressure_tape": True},
         "cg_iterations_per_solve": {("plain" if args.warm_start == "none" else args.warm_start): {k: round(v, 1) for k, v in per_solve.items()}},
         "loss": loss,

@@ -126,6 +126,7 @@ struct Collider {
 
 enum class ProjectionSolver : std::uint32_t { pcg = DS_PROJECTION_PCG, rbgs = DS_PROJECTION_RBGS };
 enum class WarmStart : std::uint32_t { none = DS_WARM_START_NONE, previous = DS_WARM_START_PREVIOUS, cnn = DS_WARM_START_CNN };
+enum class CgVariant : std::uint32_t { single_reduction = DS_CG_SINGLE_REDUCTION, two_reduction = DS_CG_TWO_REDUCTION };
 
 struct Configuration {
     std::uint32_t resolution[3]{32u, 48u, 32u};
@@ -142,6 +143,7 @@ struct Configuration {
     double cg_tolerance{1e-6};
     std::uint32_t cg_max_iterations{0u};
     WarmStart warm_start{WarmStart::none};
+    CgVariant cg_variant{CgVariant::single_reduction};
 };
 
 struct ScalarField {
@@ -288,6 +290,7 @@ inline ds_config to_abi(const Configuration& cfg, std::vector<ds_collider>& coll
     out.cg_max_iterations = cfg.cg_max_iterations;
     out.cg_tolerance = cfg.cg_tolerance;
     out.warm_start = static_cast<std::uint32_t>(cfg.warm_start);
+    out.cg_variant = static_cast<std::uint32_t>(cfg.cg_variant);
     return out;
 }
 

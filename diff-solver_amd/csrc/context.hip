@@ -97,9 +97,11 @@ void validate(const ds_config& cfg, uint32_t mode) {
         throw Failure{DS_ERROR_UNSUPPORTED, "vorticity confinement is not built (set vorticity_confinement_enabled = 0)"};
     if (cfg.projection_solver != DS_PROJECTION_PCG)
         throw Failure{DS_ERROR_UNSUPPORTED, "only the PCG projection solver runs on the GPU"};
-    if (cfg.warm_start > DS_WARM_START_PREVIOUS)
-        throw Failure{DS_ERROR_UNSUPPORTED, "warm_start must be none or previous for this entry point"};
+    if (cfg.warm_start > DS_WARM_START_CNN) bad("warm_start out of range");
+    if (cfg.warm_start == DS_WARM_START_CNN && (cfg.resolution[0] % 64 || cfg.resolution[1] % 16 || cfg.resolution[2] % 4))
+        throw Failure{DS_ERROR_UNSUPPORTED, "the warm-start network needs nx % 64 == 0, ny % 16 == 0 and nz % 4 == 0"};
     if (!(cfg.cg_tolerance >= 0.0) || !std::isfinite(cfg.cg_tolerance)) bad("cg_tolerance must be >= 0 and finite");
+    if (cfg.cg_variant > DS_CG_TWO_REDUCTION) bad("cg_variant out of range");
 }
 
 // model.cpp:56-182 plus the pressure connectivity (operators.cu:852-873): solid mask from cell
@@ -357,7 +359,12 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
         c->has_solids = d.solids > 0;
         c->has_fixed_pressure = d.fixed_pressure;
         c->mask = upload(c, d.mask);
-        c->code = upload(c, d.code);
+        {
+            // element n is a pad of 0 that out-of-domain stencil loads read
+            std::vector<uint8_t> padded(d.code);
+            padded.push_back(0);
+            c->code = upload(c, padded);
+        }
         if (d.any_fixed_term) c->fixed_term = upload(c, d.fixed_term);
         if (cfg.collider_count) {
             for (int a = 0; a < 3; ++a) c->collider_velocity[a] = upload(c, d.collider_velocity[a]);
@@ -373,9 +380,23 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
         c->pressure_lo = device_alloc<float>(c, n);
         // PCG work vectors
         c->pcg.b = device_alloc<float>(c, n);
-        c->pcg.r = device_alloc<float>(c, n);
-        c->pcg.p[0] = device_alloc<float>(c, n);
-        c->pcg.p[1] = device_alloc<float>(c, n);
+        // iteration vectors carry a zero pad element n (read by out-of-domain stencil loads)
+        c->pcg.r = device_alloc<float>(c, n + 1);
+        c->pcg.p[0] = device_alloc<float>(c, n + 1);
+        c->pcg.p[1] = device_alloc<float>(c, n + 1);
+        for (float* v : {c->pcg.r, c->pcg.p[0], c->pcg.p[1]}) fill_zero(c, v, n + 1);
+        c->pcg.variant = cfg.cg_variant;
+        if (c->pcg.variant == DS_CG_SINGLE_REDUCTION) {
+            // s is written on unknowns only; its other cells must read as 0 (as halo values)
+            c->pcg.r_alt = device_alloc<float>(c, n + 1);
+            fill_zero(c, c->pcg.r_alt, n + 1);
+            for (int k = 0; k < 2; ++k) {
+                c->pcg.wv[k] = device_alloc<float>(c, n + 1);
+                c->pcg.sv[k] = device_alloc<float>(c, n + 1);
+                fill_zero(c, c->pcg.wv[k], n + 1);
+                fill_zero(c, c->pcg.sv[k], n + 1);
+            }
+        }
         {
             // about one block per CU: zc = tiles * nz / 256 rounded up to a power of two in [4, 64]
             PlaneTiling& t = c->pcg.tiling;
@@ -388,8 +409,10 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
             t.zc = zc;
             const uint64_t G = tiles * (((uint64_t)g.nz + zc - 1) / zc);
             c->pcg.blocks = (uint32_t)((G + 7) & ~7ull);
+            // the fused kernel gives each thread at most one producer partial
+            if (c->pcg.blocks > kPcgThreads) throw Failure{DS_ERROR_UNSUPPORTED, "grid too large for the PCG tiling"};
         }
-        c->pcg.partials = device_alloc<double>(c, (size_t)kPartCount * kPcgMaxBlocks);
+        c->pcg.partials = device_alloc<double>(c, 2 * (size_t)kPartCount * kPcgMaxBlocks);   // two sets
         c->pcg.hist = device_alloc<double>(c, 2 * ((size_t)c->max_iterations + 4));
         c->pcg.status = device_alloc<PcgStatus>(c, 1);
         void* hs = nullptr;
@@ -426,6 +449,7 @@ ds_status ds_context_destroy(ds_context* c) {
         st = DS_ERROR_RUNTIME;
     }
     for (void* p : c->allocations) (void)hipFree(p);
+    unet_release(c->unet);
     if (c->pcg.host_status) (void)hipHostFree(c->pcg.host_status);
     for (auto& e : c->pcg.batch_event)
         if (e) (void)hipEventDestroy(e);

@@ -82,10 +82,12 @@ struct PcgStatus {
     double rr;              // last reduced ||r||^2 (recursive or true)
 };
 
-// HIP-event timing of sampled PCG kernel launches (kernel 0 = direction, 1 = update).
+// HIP-event timing of sampled PCG kernel launches (kernel 0 = direction, 1 = update, 2 = the
+// fused single-reduction iteration).
+constexpr int kProfiledKernels = 3;
 struct Profiler {
     uint32_t every = 0;
-    uint64_t launches[2]{};
+    uint64_t launches[kProfiledKernels]{};
     std::vector<hipEvent_t> pool;   // created lazily, reused
     size_t used = 0;
     struct Rec {
@@ -110,13 +112,20 @@ struct PlaneTiling {
     uint32_t tiles_x, tiles_y, zc;
 };
 
+struct UNet;   // warm-start network (unet.hip)
+
 struct PcgWork {
     PlaneTiling tiling{1, 1, 4};
     float* b = nullptr;          // right-hand side (zero off the unknowns)
     float* r = nullptr;          // residual
     float* p[2]{};               // search directions, ping-pong between iterations
     uint32_t blocks = 8;         // blocks of the PCG kernels
-    double* partials = nullptr;  // kPartCount * kPcgMaxBlocks
+    uint32_t variant = DS_CG_SINGLE_REDUCTION;
+    // single-reduction form: r (sets r, r_alt), w = A D^-1 r and s = A p double-buffered; p = p[0]
+    float* r_alt = nullptr;
+    float* wv[2]{};
+    float* sv[2]{};
+    double* partials = nullptr;  // two sets of kPartCount * kPcgMaxBlocks
     double* hist = nullptr;      // per-iteration scalars: rz[k], pq[k] (2 * (max_it + 2))
     PcgStatus* status = nullptr; // device
     PcgStatus* host_status = nullptr;  // pinned, 2 slots (double-buffered batches)
@@ -178,6 +187,7 @@ struct ds_context {
 
     ds::PcgWork pcg;
     ds::Profiler profiler;
+    ds::UNet* unet = nullptr;         // warm-start network, loaded by ds_warm_start_load
 
     std::vector<void*> allocations;   // everything above, released at destroy
 };
@@ -209,5 +219,12 @@ void combine_pair(ds_context* c, const float* hi, const float* lo, float* out);
 
 // Launch wrappers shared between translation units.
 void launch_apply_matrix(ds_context* c, const float* x, float* y);
+
+// Warm-start network (unet.hip).
+uint64_t unet_parameter_count();
+void unet_load(ds_context* c, const float* params, uint64_t count, double output_scale);
+// Guess for A x = b from b (device); needs the ||b||^2 partials of b in partial set 0.
+const float* unet_infer(ds_context* c, const float* b);
+void unet_release(UNet* u);
 
 }  // namespace ds

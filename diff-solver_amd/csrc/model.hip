@@ -446,17 +446,25 @@ ds_status ds_simulate(ds_context* c, const ds_state* initial, const ds_control* 
     return x.soft;
 }
 
-ds_status ds_loss_gradient(ds_context* c, const ds_state* initial, const ds_control* controls, uint32_t control_count,
-                           const ds_parameters* params, uint32_t steps, uint32_t checkpoint_interval, const float* target_density,
-                           ds_state* final_state, ds_state_adjoint* initial_adj, ds_control_adjoint* control_adj,
-                           const ds_parameter_adjoint* param_adj, ds_trajectory_info* info) {
-    if (!c || !initial || !controls || !params || !target_density || !initial_adj || steps == 0 ||
+}  // extern "C"
+
+namespace {
+
+// Checkpointed forward pass and reverse sweep (solver.ixx:115-150 with a trajectory adjoint that
+// is zero except at the final state). The terminal adjoint is either the density loss
+// 0.5 ||rho_T - target||^2 (target_density) or given (terminal_adj).
+ds_status trajectory_gradient(const char* who, ds_context* c, const ds_state* initial, const ds_control* controls,
+                              uint32_t control_count, const ds_parameters* params, uint32_t steps, uint32_t checkpoint_interval,
+                              const float* target_density, const ds_state_adjoint* terminal_adj, ds_state* final_state,
+                              ds_state_adjoint* initial_adj, ds_control_adjoint* control_adj, const ds_parameter_adjoint* param_adj,
+                              ds_trajectory_info* info) {
+    if (!c || !initial || !controls || !params || !(target_density || terminal_adj) || !initial_adj || steps == 0 ||
         (control_count != 1 && control_count != steps)) {
-        set_error("ds_loss_gradient: invalid argument (control_count must be 1 or steps; steps > 0)");
+        set_error(std::string(who) + ": invalid argument (control_count must be 1 or steps; steps > 0)");
         return DS_ERROR_INVALID_ARGUMENT;
     }
     if (c->mode != DS_MODE_DIFFERENTIABLE) {
-        set_error("ds_loss_gradient: the context was not created in differentiable mode");
+        set_error(std::string(who) + ": the context was not created in differentiable mode");
         return DS_ERROR_INVALID_ARGUMENT;
     }
     ds_trajectory_info local{};
@@ -466,23 +474,31 @@ ds_status ds_loss_gradient(ds_context* c, const ds_state* initial, const ds_cont
     ti->checkpoint_interval = checkpoint_interval ? checkpoint_interval : 1u;
     ACtx x{c, ti};
     const ds_status st = guard([&] {
-        check_state(initial, "ds_loss_gradient: initial state has a null field");
+        check_state(initial, "initial state has a null field");
         const ds_state init_view = as_state(*initial_adj);
-        check_state(&init_view, "ds_loss_gradient: initial adjoint has a null field");
+        check_state(&init_view, "initial adjoint has a null field");
+        if (terminal_adj) {
+            const ds_state tv = as_state(*terminal_adj);
+            check_state(&tv, "final-state adjoint has a null field");
+        }
         AbiModel m;
         std::function<const ds_control&(uint32_t)> ctrl = [&](uint32_t s) -> const ds_control& {
             return controls[control_count == 1 ? 0 : s];
         };
         auto t = xayah::solver::simulate_checkpointed<AbiModel, ACtx, AState, ds_control, ds_parameters>(
             m, x, borrow(*initial), ctrl, *params, steps, ti->checkpoint_interval, true);
-        // terminal adjoint and loss
+        // terminal adjoint (and loss)
         AState final_adj = m.allocate_state_adjoint(x);
-        DBuf<double> partials(c->stream, 1024), loss(c->stream, 1);
-        density_loss_kernel<<<1024, kBlock, 0, c->stream>>>(c->geo.n, t.final_state->v.density, target_density, final_adj.v.density,
-                                                            partials.p);
-        sum_partials_kernel<<<1, kBlock, 0, c->stream>>>(partials.p, 1024, loss.p);
-        check_launch("density_loss_kernel");
-        DS_HIP(hipMemcpyAsync(&ti->loss, loss.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        if (target_density) {
+            DBuf<double> partials(c->stream, 1024), loss(c->stream, 1);
+            density_loss_kernel<<<1024, kBlock, 0, c->stream>>>(c->geo.n, t.final_state->v.density, target_density, final_adj.v.density,
+                                                                partials.p);
+            sum_partials_kernel<<<1, kBlock, 0, c->stream>>>(partials.p, 1024, loss.p);
+            check_launch("density_loss_kernel");
+            DS_HIP(hipMemcpyAsync(&ti->loss, loss.p, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        } else {
+            m.copy_state_adjoint(borrow(as_state(*terminal_adj)), final_adj, x);
+        }
         if (final_state) {
             AState out = borrow(*final_state);
             m.copy_state(*t.final_state, out, x);
@@ -499,8 +515,36 @@ ds_status ds_loss_gradient(ds_context* c, const ds_state* initial, const ds_cont
         DS_HIP(hipStreamSynchronize(c->stream));
     });
     if (st != DS_OK) return st;
-    if (x.soft != DS_OK) set_error("ds_loss_gradient: a pressure solve did not converge");
+    if (x.soft != DS_OK) set_error(std::string(who) + ": a pressure solve did not converge");
     return x.soft;
+}
+
+}  // namespace
+
+extern "C" {
+
+ds_status ds_loss_gradient(ds_context* c, const ds_state* initial, const ds_control* controls, uint32_t control_count,
+                           const ds_parameters* params, uint32_t steps, uint32_t checkpoint_interval, const float* target_density,
+                           ds_state* final_state, ds_state_adjoint* initial_adj, ds_control_adjoint* control_adj,
+                           const ds_parameter_adjoint* param_adj, ds_trajectory_info* info) {
+    if (!target_density) {
+        set_error("ds_loss_gradient: null target");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return trajectory_gradient("ds_loss_gradient", c, initial, controls, control_count, params, steps, checkpoint_interval,
+                               target_density, nullptr, final_state, initial_adj, control_adj, param_adj, info);
+}
+
+ds_status ds_trajectory_vjp(ds_context* c, const ds_state* initial, const ds_control* controls, uint32_t control_count,
+                            const ds_parameters* params, uint32_t steps, uint32_t checkpoint_interval,
+                            const ds_state_adjoint* final_adj, ds_state* final_state, ds_state_adjoint* initial_adj,
+                            ds_control_adjoint* control_adj, const ds_parameter_adjoint* param_adj, ds_trajectory_info* info) {
+    if (!final_adj) {
+        set_error("ds_trajectory_vjp: null final-state adjoint");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return trajectory_gradient("ds_trajectory_vjp", c, initial, controls, control_count, params, steps, checkpoint_interval,
+                               nullptr, final_adj, final_state, initial_adj, control_adj, param_adj, info);
 }
 
 }  // extern "C"

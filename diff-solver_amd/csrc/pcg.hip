@@ -12,11 +12,11 @@
 // residual above 1e-6 ||b|| on smooth right-hand sides). Stopping rule ||b - A x|| <= tol ||b||
 // with the true residual recomputed every `replacement_interval` iterations and at exit.
 //
-// Kernels per iteration (grid-stride over contiguous per-block cell ranges, XCD-grouped):
-//   direction: p_k = r/d + beta p_{k-1} built on the fly at the centre and its 6 neighbours,
-//              q = A p_k, partial <p_k, q>         (reads r, p_{k-1}, code; writes p_k)
-//   update:    q = A p_k again, r -= alpha q, x += alpha p_k, partials <r,z>, <r,r>
-//              (reads r, p_k, code, x; writes r, x)
+// Two recurrences (ds_config.cg_variant):
+//   single reduction (default, pcg_fused.hpp): one fused kernel per iteration;
+//   two reductions (pcg_plane.hpp): per iteration
+//     direction: p_k = r/d + beta p_{k-1}, q = A p_k, partial <p_k, q>   (reads r, p_{k-1}; writes p_k)
+//     update:    q = A p_k again, r -= alpha q, x += alpha p_k, partials <r,z>, <r,r>
 // Each consumer finishes its producer's per-block FP64 partials in a fixed order, so every
 // block derives bit-identical scalars and the iteration is deterministic without atomics.
 #include <hip/hip_runtime.h>
@@ -26,6 +26,7 @@
 #include <cstdio>
 
 #include "context.hpp"
+#include "pcg_fused.hpp"
 #include "pcg_plane.hpp"
 
 namespace ds {
@@ -210,6 +211,38 @@ __global__ __launch_bounds__(1024) void pcg_true_residual_kernel(Geo g, uint32_t
     }
 }
 
+// Residual replacement of the single-reduction form: r = b - A (x_hi + x_lo) and s = A p in FP64,
+// rounded to FP32 (with_s = 0 when no direction exists yet); partial ||r||^2 into out_parts.
+__global__ __launch_bounds__(1024) void pcg_fused_replace_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
+                                                                const float* __restrict__ b, const float* __restrict__ xh,
+                                                                const float* __restrict__ xl, const float* __restrict__ p,
+                                                                float* __restrict__ r, float* __restrict__ s, uint32_t with_s,
+                                                                double* __restrict__ out_parts, PcgStatus* status,
+                                                                uint32_t respect_stop) {
+    __shared__ double lds[3 * kMaxWaves];
+    if (respect_stop && status->stop) return;
+    uint32_t lb, start, end;
+    block_range(g.n, lb, start, end);
+    double rr = 0.0;
+    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+        const uint32_t d = code[i];
+        if (d == 0u) continue;
+        int x, y, z;
+        decode_cell(g, i, x, y, z);
+        auto xv = [&](uint32_t j) -> double { return (double)xh[j] + (double)xl[j]; };
+        const double ri = (double)b[i] - ((double)d * xv(i) - neighbour_sum<double>(g, per, x, y, z, i, xv));
+        r[i] = (float)ri;
+        rr += ri * ri;
+        if (with_s) {
+            auto pv = [&](uint32_t j) -> double { return code[j] ? (double)p[j] : 0.0; };
+            s[i] = (float)((double)d * pv(i) - neighbour_sum<double>(g, per, x, y, z, i, pv));
+        }
+    }
+    double v[1]{rr};
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) out_parts[kPartRR * kPcgMaxBlocks + lb] = v[0];
+}
+
 // One block: reduce the ||b||^2 (optional) and ||r||^2 partials into the status.
 __global__ __launch_bounds__(256) void pcg_finalize_kernel(const double* __restrict__ partials, uint32_t G,
                                                            PcgStatus* status, uint32_t with_bb) {
@@ -287,6 +320,132 @@ PcgStatus read_status(ds_context* c) {
     DS_HIP(hipMemcpyAsync(&c->pcg.host_status[0], c->pcg.status, sizeof(PcgStatus), hipMemcpyDeviceToHost, c->stream));
     DS_HIP(hipStreamSynchronize(c->stream));
     return c->pcg.host_status[0];
+}
+
+template <int Mode>
+void launch_fused(ds_context* c, uint32_t per, const FusedVecs& v, const double* in_parts, const double* bb_parts,
+                  double* out_parts, double tol2, uint32_t k, uint32_t check_only, uint32_t respect_stop) {
+    PcgWork& w = c->pcg;
+    pcg_fused_kernel<Mode><<<w.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, w.tiling, c->code, v, in_parts, bb_parts, out_parts,
+                                                                    w.hist, w.status, tol2, k, check_only, respect_stop);
+}
+
+// Single-reduction iteration from the state left by launch_setup (r in set 0, BB/RR/RZ partials
+// in partial set 0) until the true residual meets tol or max_it.
+ds_solve_info pcg_iterate_fused(ds_context* c, float* xh, float* xl, double tol, uint32_t max_it, bool warm, bool rejected) {
+    PcgWork& w = c->pcg;
+    const uint32_t per = periodic_bits(c);
+    const double tol2 = tol * tol;
+    const uint32_t R = c->replacement_interval;
+    float* rv[2]{w.r, w.r_alt};
+    double* parts[2]{w.partials, w.partials + (size_t)kPartCount * kPcgMaxBlocks};
+    auto vecs = [&](int set, bool prime) {
+        FusedVecs v;
+        v.r = rv[set];
+        v.w = w.wv[set];
+        v.s = w.sv[set];
+        v.r_out = rv[set ^ 1];
+        v.w_out = prime ? w.wv[set] : w.wv[set ^ 1];
+        v.s_out = w.sv[set ^ 1];
+        v.p = w.p[0];
+        v.xh = xh;
+        v.xl = xl;
+        return v;
+    };
+    auto prime = [&](int set, uint32_t respect_stop) {
+        launch_fused<kFusedPrime>(c, per, vecs(set, true), nullptr, nullptr, parts[set], tol2, 0u, 0u, respect_stop);
+    };
+    auto replace = [&](int set, bool with_s, uint32_t respect_stop) {
+        pcg_fused_replace_kernel<<<w.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, c->code, w.b, xh, xl, w.p[0], rv[set], w.sv[set],
+                                                                          with_s ? 1u : 0u, parts[set], w.status, respect_stop);
+    };
+    ds_solve_info info{};
+    info.warm_started = warm ? 1u : 0u;
+    info.guess_rejected = rejected ? 1u : 0u;
+    int set = 0;
+    prime(0, 0u);
+    check_launch("pcg_fused_kernel (prime)");
+    uint32_t k = 0;
+    bool first = true;
+    uint32_t restarts = 0;
+    Profiler& prof = c->profiler;
+    while (true) {
+        uint32_t batch = 1;
+        int slot = 0, pending = -1;
+        bool stop_seen = false;
+        while (k < max_it && !stop_seen) {
+            const uint32_t nb = std::min(batch, max_it - k);
+            for (uint32_t j = 0; j < nb; ++j, ++k) {
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (prof.every && k % prof.every == 0 && prof.used + 2 <= 6000) {
+                    e0 = prof.take();
+                    e1 = prof.take();
+                }
+                if (e0) DS_HIP(hipEventRecord(e0, c->stream));
+                const FusedVecs v = vecs(set, false);
+                const double* bbp = first ? parts[0] : nullptr;
+                if (k == 0)
+                    launch_fused<kFusedFirst>(c, per, v, parts[set], bbp, parts[set ^ 1], tol2, k, 0u, 1u);
+                else
+                    launch_fused<kFusedNext>(c, per, v, parts[set], bbp, parts[set ^ 1], tol2, k, 0u, 1u);
+                first = false;
+                if (e1) {
+                    DS_HIP(hipEventRecord(e1, c->stream));
+                    prof.recs.push_back({2, e0, e1});
+                }
+                if (prof.every) prof.launches[2]++;
+                set ^= 1;
+                if ((k + 1u) % R == 0u && k + 1u < max_it) {
+                    replace(set, true, 1u);
+                    prime(set, 1u);
+                }
+            }
+            check_launch("pcg_fused_kernel");
+            DS_HIP(hipMemcpyAsync(&w.host_status[slot], w.status, sizeof(PcgStatus), hipMemcpyDeviceToHost, c->stream));
+            DS_HIP(hipEventRecord(w.batch_event[slot], c->stream));
+            if (pending >= 0) {
+                DS_HIP(hipEventSynchronize(w.batch_event[pending]));
+                if (w.host_status[pending].stop) stop_seen = true;
+            }
+            pending = slot;
+            slot ^= 1;
+            batch = std::min<uint32_t>(batch * 2u, 32u);
+        }
+        PcgStatus st = read_status(c);
+        if (!st.stop && k >= max_it) {
+            // one more convergence test on the last iteration's residual
+            if (k == 0)
+                launch_fused<kFusedFirst>(c, per, vecs(set, false), parts[set], first ? parts[0] : nullptr, parts[set ^ 1], tol2, k, 1u, 1u);
+            else
+                launch_fused<kFusedNext>(c, per, vecs(set, false), parts[set], first ? parts[0] : nullptr, parts[set ^ 1], tol2, k, 1u, 1u);
+            first = false;
+            check_launch("pcg_fused_kernel (check)");
+            st = read_status(c);
+        }
+        // the launches queued after the stopping iteration returned at once: `set` is one flip per
+        // launch ahead; the stopping iteration read set (done & 1) relative to this pass's start
+        const uint32_t done = st.stop ? st.iterations : max_it;
+        set = (int)((uint32_t)set ^ ((k - done) & 1u));
+        k = done;
+        replace(set, done > 0, 0u);
+        pcg_finalize_kernel<<<1, kBlock, 0, c->stream>>>(parts[set], w.blocks, w.status, 0u);
+        check_launch("pcg_fused_replace_kernel");
+        const PcgStatus fin = read_status(c);
+        info.iterations = done;
+        info.rhs_norm = std::sqrt(fin.bb);
+        info.final_residual = std::sqrt(std::max(0.0, fin.rr));
+        const bool finite = std::isfinite(fin.rr);
+        const bool ok = finite && fin.rr <= tol2 * fin.bb;
+        info.converged = ok ? 1u : 0u;
+        if (!finite) info.final_residual = fin.rr;
+        if (ok || !finite || !st.stop || done >= max_it || ++restarts > 8) break;
+        // the recursive residual met the tolerance but the true one does not: continue from the
+        // true residual (r and s replaced above) with the same search direction
+        prime(set, 0u);
+        pcg_clear_stop_kernel<<<1, 64, 0, c->stream>>>(w.status);
+        check_launch("pcg_clear_stop_kernel");
+    }
+    return info;
 }
 
 // Iterates from the state left by launch_setup until the true residual meets tol or max_it.
@@ -389,9 +548,16 @@ ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], c
         in.u[a] = u ? u[a] : nullptr;
         in.extra[a] = extra ? extra[a] : nullptr;
     }
+    max_it = std::min(max_it, c->max_iterations);   // the per-iteration history is sized for this cap
+    if (x0h == nullptr && c->config.warm_start == DS_WARM_START_CNN) {
+        // the network's guess: b first (set-up without a guess), then the set-up again from the guess
+        launch_setup(c, kind, in, nullptr, nullptr, false, xh, xl);
+        x0h = unet_infer(c, c->pcg.b);
+        x0l = nullptr;
+        kind = kRhsGiven;
+    }
     const bool warm = x0h != nullptr;
     bool rejected = false;
-    max_it = std::min(max_it, c->max_iterations);   // the per-iteration history is sized for this cap
     launch_setup(c, kind, in, x0h, x0l, x0h == xh, xh, xl);
     double rr0 = 0.0, bb = 0.0;
     if (warm) {
@@ -406,7 +572,8 @@ ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], c
             launch_setup(c, kRhsGiven, in, nullptr, nullptr, false, xh, xl);
         }
     }
-    ds_solve_info info = pcg_iterate(c, xh, xl, tol, max_it, warm, rejected);
+    ds_solve_info info = c->pcg.variant == DS_CG_SINGLE_REDUCTION ? pcg_iterate_fused(c, xh, xl, tol, max_it, warm, rejected)
+                                                                  : pcg_iterate(c, xh, xl, tol, max_it, warm, rejected);
     info.initial_residual = warm && !rejected ? std::sqrt(std::max(0.0, rr0)) : info.rhs_norm;
     (void)bb;
     return info;
@@ -462,7 +629,7 @@ ds_status ds_profile_enable(ds_context* c, uint32_t sample_every) {
         return DS_ERROR_INVALID_ARGUMENT;
     }
     c->profiler.every = sample_every;
-    c->profiler.launches[0] = c->profiler.launches[1] = 0;
+    for (auto& l : c->profiler.launches) l = 0;
     return DS_OK;
 }
 
@@ -474,26 +641,58 @@ ds_status ds_profile_read(ds_context* c, ds_kernel_timing* out, uint32_t capacit
     return guard([&] {
         DS_HIP(hipStreamSynchronize(c->stream));
         Profiler& prof = c->profiler;
-        double total[2]{0.0, 0.0};
-        uint64_t samples[2]{0, 0};
+        double total[kProfiledKernels]{};
+        uint64_t samples[kProfiledKernels]{};
         for (const Profiler::Rec& r : prof.recs) {
             float ms = 0.0f;
             DS_HIP(hipEventElapsedTime(&ms, r.a, r.b));
             total[r.kernel] += ms;
             samples[r.kernel]++;
         }
-        const char* names[2]{"pcg_direction_kernel", "pcg_update_kernel"};
-        const uint32_t n = std::min<uint32_t>(capacity, 2u);
-        for (uint32_t k = 0; k < n; ++k) {
-            std::snprintf(out[k].name, sizeof(out[k].name), "%s", names[k]);
-            out[k].launches = prof.launches[k];
-            out[k].samples = samples[k];
-            out[k].total_ms = total[k];
+        const char* names[kProfiledKernels]{"pcg_direction_kernel", "pcg_update_kernel", "pcg_fused_kernel"};
+        uint32_t n = 0;
+        for (int k = 0; k < kProfiledKernels; ++k) {   // kernels that ran
+            if (prof.launches[k] == 0 || n >= capacity) continue;
+            std::snprintf(out[n].name, sizeof(out[n].name), "%s", names[k]);
+            out[n].launches = prof.launches[k];
+            out[n].samples = samples[k];
+            out[n].total_ms = total[k];
+            ++n;
         }
         if (count) *count = n;
         prof.recs.clear();
         prof.used = 0;
-        prof.launches[0] = prof.launches[1] = 0;
+        for (auto& l : prof.launches) l = 0;
+    });
+}
+
+uint64_t ds_warm_start_parameter_count(void) { return unet_parameter_count(); }
+
+ds_status ds_warm_start_load(ds_context* c, const float* params, uint64_t count, double output_scale) {
+    if (!c || !params) {
+        set_error("ds_warm_start_load: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    if (!(output_scale > 0.0) || !std::isfinite(output_scale)) {
+        set_error("ds_warm_start_load: output_scale must be positive and finite");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] { unet_load(c, params, count, output_scale); });
+}
+
+ds_status ds_warm_start_infer(ds_context* c, const float* b, float* x) {
+    if (!c || !b || !x) {
+        set_error("ds_warm_start_infer: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] {
+        DS_HIP(hipMemcpyAsync(c->pcg.b, b, (size_t)c->geo.n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        RhsIn in{};
+        launch_setup(c, kRhsGiven, in, nullptr, nullptr, false, c->pressure_hi, c->pressure_lo);   // ||b||^2 partials
+        c->pressure_valid = false;
+        const float* g = unet_infer(c, c->pcg.b);
+        DS_HIP(hipMemcpyAsync(x, g, (size_t)c->geo.n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        DS_HIP(hipStreamSynchronize(c->stream));
     });
 }
 

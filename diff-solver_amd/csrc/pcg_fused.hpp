@@ -1,0 +1,272 @@
+// pcg_fused.hpp — single-reduction Jacobi-PCG: one kernel launch per iteration.
+//
+// The Chronopoulos-Gear arrangement of preconditioned CG (Chronopoulos & Gear, "s-step iterative
+// methods for symmetric linear systems", J. Comput. Appl. Math. 25, 1989) carries w = A u
+// (u = D^-1 r) and s = A p as vectors, so both inner products of an iteration,
+// gamma = <r, u> and delta = <w, u>, come from the same vectors and need one global reduction:
+//
+//   beta_i  = gamma_i / gamma_{i-1}                      (0 for i = 0)
+//   alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1})   (gamma_0 / delta_0 for i = 0)
+//   p_i = u_i + beta_i p_{i-1};   s_i = w_i + beta_i s_{i-1}
+//   x += alpha_i p_i;   r_{i+1} = r_i - alpha_i s_i;   u_{i+1} = D^-1 r_{i+1};   w_{i+1} = A u_{i+1}
+//   partials of gamma_{i+1}, delta_{i+1} and rho_{i+1} = ||r_{i+1}||^2
+//
+// In exact arithmetic the iterates equal those of the two-reduction form. One kernel does all of
+// iteration i: every quantity is pointwise except w_{i+1} = A u_{i+1}, and u_{i+1} at a neighbour
+// is a pointwise function of that neighbour's r_i, w_i, s_{i-1} and code, so each block rebuilds
+// the u_{i+1} of its tile rim and halo planes from those vectors. r, w and s are therefore
+// double-buffered (set c is read, set c^1 written); p and x are read and written by their own
+// cell only and stay single. The per-block FP64 partials are double-buffered the same way.
+//
+// Layout: the plane march of the two-kernel path — a 1024-thread block owns a 64 x 16 (x, y)
+// tile and a chunk of z planes; u of the plane being differentiated sits in LDS (own cell plus
+// the tile rim), its z neighbours in registers. The raw loads of plane z+1 are consumed entirely
+// while plane z is computed and those of plane z+3 are issued then; the z loop is unrolled by two
+// over two named load slots, so a slot is never copied and the loads stay in flight for two
+// plane steps.
+#pragma once
+
+#include "pcg_plane.hpp"
+
+namespace ds {
+
+// Kernel modes: the first iteration after set-up (no previous direction), a general iteration,
+// and the priming pass w = A u, partials gamma, delta, rho (after set-up or a residual replacement).
+enum FusedMode : int { kFusedFirst = 0, kFusedNext = 1, kFusedPrime = 2 };
+
+struct FusedVecs {
+    const float* r;       // r_i          (set c)
+    const float* w;       // w_i = A u_i  (set c)
+    const float* s;       // s_{i-1}      (set c)
+    float* r_out;         // r_{i+1}      (set c^1)
+    float* w_out;         // w_{i+1}      (set c^1; the priming pass writes set c)
+    float* s_out;         // s_i          (set c^1)
+    float* p;             // p_{i-1} -> p_i (in place)
+    float* xh;            // solution pair (in place)
+    float* xl;
+};
+
+// Halo values of one cell: enough to form u_{i+1} there.
+struct FHalo {
+    float r, w, s;
+    uint8_t d;   // kept as loaded: a widening right after the load would wait for it
+};
+// Own-cell values only the owning thread needs.
+struct FOwn {
+    float p, h, l;
+};
+struct FPlane {
+    FHalo c, x1, y2;
+    FOwn o;
+};
+
+// Loaded values are not touched before the plane step that consumes them: a select or a
+// divergent branch on a load result makes the compiler wait for that load on the spot, which
+// would drain the prefetch. Cells outside the domain (and the rim slots of lanes that are not on
+// the tile edge) therefore read element n of every vector and of the code, a pad that is always 0.
+__device__ __forceinline__ uint32_t halo_index(const Geo& g, int64_t j) { return j >= 0 ? (uint32_t)j : (uint32_t)g.n; }
+
+template <int Mode>
+__device__ __forceinline__ FHalo fused_load_halo(uint32_t j, const FusedVecs& v, const uint8_t* __restrict__ code) {
+    FHalo h;
+    h.r = v.r[j];
+    h.d = code[j];
+    h.w = Mode != kFusedPrime ? v.w[j] : 0.0f;
+    h.s = Mode == kFusedNext ? v.s[j] : 0.0f;
+    return h;
+}
+
+// s_i = w_i + beta s_{i-1} and r_{i+1} = r_i - alpha s_i with explicit FMAs, so that the owner of
+// a cell and a block that rebuilds it as halo round identically.
+template <int Mode>
+__device__ __forceinline__ float fused_s(const FHalo& h, float beta) {
+    return Mode == kFusedNext ? __fmaf_rn(beta, h.s, h.w) : h.w;
+}
+template <int Mode>
+__device__ __forceinline__ float fused_r(const FHalo& h, float alpha, float beta) {
+    return Mode == kFusedPrime ? h.r : __fmaf_rn(-alpha, fused_s<Mode>(h, beta), h.r);
+}
+template <int Mode>
+__device__ __forceinline__ float fused_u(const FHalo& h, float alpha, float beta) {
+    return fused_r<Mode>(h, alpha, beta) * inv_diag(h.d);
+}
+
+// Iteration k (or the priming pass). in_parts / out_parts: partial sets (kPartCount * kPcgMaxBlocks
+// doubles each, slots RR = rho, RZ = gamma, PQ = delta; BB read from bb_parts on the first launch
+// of a solve). hist[2k] = gamma_k, hist[2k+1] = alpha_k. check_only: convergence test only.
+template <int Mode>
+__global__ __launch_bounds__(1024) void pcg_fused_kernel(Geo g, uint32_t per, PlaneTiling pt, const uint8_t* __restrict__ code,
+                                                         FusedVecs v, const double* __restrict__ in_parts,
+                                                         const double* __restrict__ bb_parts, double* __restrict__ out_parts,
+                                                         double* __restrict__ hist, PcgStatus* status, double tol2, uint32_t k,
+                                                         uint32_t check_only, uint32_t respect_stop) {
+    __shared__ float tile[2][kTY + 2][kTX + 2];
+    __shared__ double lds[4 * kMaxWaves];
+    if ((Mode != kFusedPrime || respect_stop) && status->stop) return;
+    const uint32_t lb = xcd_block();
+    int x0, y0, z0, z1;
+    plane_block(g, pt, lb, x0, y0, z0, z1);
+    const int lx = (int)(threadIdx.x & 63u), ly = (int)(threadIdx.x >> 6);
+    const int x = x0 + lx, y = y0 + ly;
+    const bool own = x < g.nx && y < g.ny;
+    const Slots sl = make_slots(x0, y0, lx, ly);
+    // the y rim belongs to whole waves (rows 0 and kTY-1): a wave-uniform branch
+    const int wave_row = __builtin_amdgcn_readfirstlane(ly);
+    const bool row_edge = wave_row == 0 || wave_row == kTY - 1;
+    constexpr bool kUpdate = Mode != kFusedPrime;
+
+    // this thread's share of the producer's partials, loaded before the planes so that waiting
+    // for them leaves the plane loads in flight (gridDim.x <= blockDim.x)
+    const bool has_part = threadIdx.x < gridDim.x;
+    const uint32_t tp = has_part ? threadIdx.x : 0u;
+    const double* bbp = bb_parts ? bb_parts : in_parts;
+    double part[4]{0.0, 0.0, 0.0, 0.0};
+    if (Mode != kFusedPrime) {
+        part[0] = in_parts[kPartRR * kPcgMaxBlocks + tp];
+        part[1] = in_parts[kPartRZ * kPcgMaxBlocks + tp];
+        part[2] = in_parts[kPartPQ * kPcgMaxBlocks + tp];
+        part[3] = bbp[kPartBB * kPcgMaxBlocks + tp];
+    }
+
+    // loads of plane zz: the own column up to the top halo plane z1 (planes beyond read the pad);
+    // rims for the chunk planes; the own-cell update values for every plane (used only where the
+    // thread owns a cell of a chunk plane)
+    auto load_plane = [&](int zz, FPlane& P) {
+        const bool chunk = zz < z1;   // block-uniform
+        P.x1 = fused_load_halo<Mode>(chunk && sl.has1 ? halo_index(g, wrapped_index(g, per, sl.sx1, y, zz)) : (uint32_t)g.n, v, code);
+        if (chunk && row_edge)
+            P.y2 = fused_load_halo<Mode>(halo_index(g, wrapped_index(g, per, x, sl.sy2, zz)), v, code);
+        else
+            P.y2 = FHalo{0.0f, 0.0f, 0.0f, 0u};
+        const int64_t j = zz <= z1 ? wrapped_index(g, per, x, y, zz) : -1;
+        P.c = fused_load_halo<Mode>(halo_index(g, j), v, code);
+        if (kUpdate) {
+            const uint32_t jo = j >= 0 ? (uint32_t)j : 0u;
+            P.o.h = v.xh[jo];
+            P.o.l = v.xl[jo];
+            P.o.p = Mode == kFusedNext ? v.p[jo] : 0.0f;
+        } else {
+            P.o = FOwn{0.0f, 0.0f, 0.0f};
+        }
+    };
+
+    FHalo below_raw{0.0f, 0.0f, 0.0f, 0u};
+    FPlane cur, slot_a, slot_b;
+    if (z0 < z1) {
+        below_raw = fused_load_halo<Mode>(halo_index(g, wrapped_index(g, per, x, y, z0 - 1)), v, code);
+        load_plane(z0, cur);
+        load_plane(z0 + 1, slot_a);
+        load_plane(z0 + 2, slot_b);
+    }
+
+    float alpha = 0.0f, beta = 0.0f;
+    if (Mode != kFusedPrime) {
+        double red[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) red[q] = has_part ? part[q] : 0.0;
+        block_sum<4>(red, lds);
+        const double rho = red[0], gamma = red[1], delta = red[2];
+        const double bb = bb_parts ? red[3] : status->bb;
+        if (!(rho > tol2 * bb) || check_only) {   // converged (NaN also stops; the true residual reports it)
+            if (lb == 0 && threadIdx.x == 0) {
+                if (bb_parts) status->bb = bb;
+                status->rr = rho;
+                status->iterations = k;
+                if (!(rho > tol2 * bb)) status->stop = 1u;
+            }
+            return;
+        }
+        double b64 = 0.0, a64;
+        if (Mode == kFusedNext) {
+            b64 = gamma / hist[2 * (k - 1)];
+            a64 = gamma / (delta - b64 * gamma / hist[2 * (k - 1) + 1]);
+        } else {
+            a64 = gamma / delta;
+        }
+        if (lb == 0 && threadIdx.x == 0) {
+            if (bb_parts) status->bb = bb;
+            hist[2 * k] = gamma;
+            hist[2 * k + 1] = a64;
+        }
+        alpha = (float)a64;
+        beta = (float)b64;
+    }
+
+    double rho = 0.0, gam = 0.0, del = 0.0;
+    if (z0 < z1) {
+        // pointwise part of plane zz for the own cell: updates, stores, gamma/rho partials; returns u_{i+1}
+        auto own_update = [&](const FPlane& P, int zz) -> float {
+            const float u = fused_u<Mode>(P.c, alpha, beta);
+            if (own && zz < z1) {
+                const uint32_t i = cidx(g, x, y, zz);
+                const float rn = fused_r<Mode>(P.c, alpha, beta);
+                if (kUpdate && P.c.d != 0u) {
+                    const float si = fused_s<Mode>(P.c, beta);
+                    const float ui = P.c.r * inv_diag(P.c.d);
+                    const float pi = Mode == kFusedNext ? __fmaf_rn(beta, P.o.p, ui) : ui;
+                    v.p[i] = pi;
+                    float h = P.o.h, l = P.o.l;
+                    compensated_add(h, l, alpha * pi);
+                    v.xh[i] = h;
+                    v.xl[i] = l;
+                    v.s_out[i] = si;
+                }
+                if (kUpdate) v.r_out[i] = P.c.d != 0u ? rn : 0.0f;
+                rho += (double)rn * rn;
+                gam += (double)rn * u;
+            }
+            return u;
+        };
+        auto stage = [&](const FPlane& P, int b) {
+            if (sl.has1) tile[b][ly + 1][sl.t1x] = fused_u<Mode>(P.x1, alpha, beta);
+            if (sl.has2) tile[b][sl.t2y][lx + 1] = fused_u<Mode>(P.y2, alpha, beta);
+        };
+        float below = fused_u<Mode>(below_raw, alpha, beta);
+        float centre = own_update(cur, z0);
+        uint8_t d = cur.c.d;
+        tile[0][ly + 1][lx + 1] = centre;
+        stage(cur, 0);
+        __syncthreads();
+        int buf = 0;
+        // one plane step: consume the slot holding plane z+1, finish plane z, refill the slot with
+        // z+3. A step with z >= z1 (the second step of an odd-length chunk) only loads and waits,
+        // so that both steps of the unrolled loop run unconditionally.
+        auto step = [&](FPlane& S, int z) {
+            const float next = own_update(S, z + 1);
+            if (z + 1 < z1) {
+                tile[buf ^ 1][ly + 1][lx + 1] = next;
+                stage(S, buf ^ 1);
+            }
+            if (own && z < z1) {
+                const uint32_t i = cidx(g, x, y, z);
+                float wv = 0.0f;
+                if (d != 0u)
+                    wv = (float)d * centre -
+                         (((tile[buf][ly + 1][lx] + tile[buf][ly + 1][lx + 2]) + (tile[buf][ly][lx + 1] + tile[buf][ly + 2][lx + 1])) +
+                          below + next);
+                v.w_out[i] = wv;
+                del += (double)wv * centre;
+            }
+            const uint8_t dn = S.c.d;
+            load_plane(z + 3, S);
+            __syncthreads();
+            below = centre;
+            centre = next;
+            d = dn;
+            buf ^= 1;
+        };
+        for (int z = z0; z < z1; z += 2) {
+            step(slot_a, z);
+            step(slot_b, z + 1);
+        }
+    }
+    double red[3]{rho, gam, del};
+    block_sum<3>(red, lds);
+    if (threadIdx.x == 0) {
+        out_parts[kPartRR * kPcgMaxBlocks + lb] = red[0];
+        out_parts[kPartRZ * kPcgMaxBlocks + lb] = red[1];
+        out_parts[kPartPQ * kPcgMaxBlocks + lb] = red[2];
+    }
+}
+
+}  // namespace ds

@@ -17,9 +17,17 @@ namespace ds {
 constexpr int kTX = kPlaneTileX;
 constexpr int kTY = kPlaneTileY;
 
+// 1/d for the diagonal counts 1..6 of the 7-point matrix, 0 off the unknowns: a chain of selects
+// (a nested conditional expression compiles to divergent branches).
 __device__ __forceinline__ float inv_diag(uint32_t d) {
-    return d == 1u ? 1.0f : d == 2u ? 0.5f : d == 3u ? (1.0f / 3.0f) : d == 4u ? 0.25f : d == 5u ? 0.2f
-                                                                                  : d == 6u ? (1.0f / 6.0f) : 0.0f;
+    float v = 0.0f;
+    v = d == 1u ? 1.0f : v;
+    v = d == 2u ? 0.5f : v;
+    v = d == 3u ? (1.0f / 3.0f) : v;
+    v = d == 4u ? 0.25f : v;
+    v = d == 5u ? 0.2f : v;
+    v = d == 6u ? (1.0f / 6.0f) : v;
+    return v;
 }
 
 // x_hi + x_lo += y with FP32 error-free transformations (Knuth two-sum, then renormalise).

@@ -9,6 +9,8 @@ import ctypes as C
 import os
 
 DS_OK = 0
+(DS_ERROR_INVALID_ARGUMENT, DS_ERROR_OUT_OF_MEMORY, DS_ERROR_LAUNCH, DS_ERROR_NOT_CONVERGED, DS_ERROR_NON_FINITE,
+ DS_ERROR_RUNTIME, DS_ERROR_UNSUPPORTED) = range(1, 8)
 STATUS_NAMES = {
     0: "ok", 1: "invalid argument", 2: "out of memory", 3: "launch failure",
     4: "not converged", 5: "non-finite", 6: "runtime error", 7: "unsupported",
@@ -20,6 +22,7 @@ SHAPE_ELLIPSOID, SHAPE_BOX = 0, 1
 PROJECTION_PCG, PROJECTION_RBGS = 0, 1
 WARM_START_NONE, WARM_START_PREVIOUS, WARM_START_CNN = 0, 1, 2
 MODE_FORWARD, MODE_DIFFERENTIABLE = 0, 1
+CG_SINGLE_REDUCTION, CG_TWO_REDUCTION = 0, 1
 
 FloatP = C.POINTER(C.c_float)
 DoubleP = C.POINTER(C.c_double)
@@ -56,6 +59,7 @@ class Config(C.Structure):
         ("cg_tolerance", C.c_double),
         ("warm_start", C.c_uint32),
         ("residual_replacement_interval", C.c_uint32),
+        ("cg_variant", C.c_uint32),
     ]
 
 

@@ -99,7 +99,7 @@ class Scene:
 
     def config(self, tolerance: float = 1e-6, max_iterations: int = 0, solver: int = abi.PROJECTION_PCG,
                warm_start: int = abi.WARM_START_NONE, rbgs_iterations: int = 80,
-               replacement_interval: int = 0) -> abi.Config:
+               replacement_interval: int = 0, cg_variant: int = abi.CG_SINGLE_REDUCTION) -> abi.Config:
         cfg = abi.Config()
         cfg.resolution[:] = list(self.resolution)
         cfg.cell_size = self.cell_size
@@ -125,6 +125,7 @@ class Scene:
         cfg.cg_tolerance = tolerance
         cfg.warm_start = warm_start
         cfg.residual_replacement_interval = replacement_interval
+        cfg.cg_variant = cg_variant
         return cfg
 
 

@@ -29,6 +29,10 @@ def load_library(path: str | None = None):
     global _LIB
     if _LIB is not None:
         return _LIB
+    # torch first: the library then binds to the HIP runtime torch has loaded (one runtime per
+    # process; loading the library first can leave torch and the library on different runtimes)
+    import torch  # noqa: F401
+
     path = path or abi.library_path()
     if not os.path.exists(path):
         raise RuntimeError(f"HIP smoke library not found at {path}; run __graft_entry__.build() first")
@@ -85,6 +89,13 @@ def load_library(path: str | None = None):
                                          C.POINTER(abi.Parameters), C.c_uint32, C.c_uint32, P, C.POINTER(abi.State),
                                          C.POINTER(abi.StateAdjoint), C.POINTER(abi.ControlAdjoint),
                                          C.POINTER(abi.ParameterAdjoint), C.POINTER(abi.TrajectoryInfo)]),
+        "ds_warm_start_parameter_count": (C.c_uint64, []),
+        "ds_warm_start_load": (C.c_int32, [P, P, C.c_uint64, C.c_double]),
+        "ds_warm_start_infer": (C.c_int32, [P, P, P]),
+        "ds_trajectory_vjp": (C.c_int32, [P, C.POINTER(abi.State), C.POINTER(abi.Control), C.c_uint32,
+                                          C.POINTER(abi.Parameters), C.c_uint32, C.c_uint32, C.POINTER(abi.StateAdjoint),
+                                          C.POINTER(abi.State), C.POINTER(abi.StateAdjoint), C.POINTER(abi.ControlAdjoint),
+                                          C.POINTER(abi.ParameterAdjoint), C.POINTER(abi.TrajectoryInfo)]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -300,6 +311,22 @@ class Context:
             self._check(st, "ds_loss_gradient")
         return info.loss, initial_adj, info
 
+    def trajectory_vjp(self, initial, control, params, steps, checkpoint_interval, final_adj, initial_adj=None,
+                       control_adj=None, param_adj=None, final_state=None, check=True):
+        """Reverse sweep for a given adjoint of the final state (accumulates into the adjoints)."""
+        initial_adj = initial_adj or self.allocate_state_adjoint()
+        info = abi.TrajectoryInfo()
+        ca = C.byref(self.control_view(control_adj)) if control_adj is not None else None
+        pa = C.byref(self.param_adjoint_view(param_adj)) if param_adj is not None else None
+        fs = C.byref(self.state_view(final_state)) if final_state is not None else None
+        st = self.lib.ds_trajectory_vjp(self.h, C.byref(self.state_view(initial)), C.byref(self.control_view(control)), 1,
+                                        C.byref(self.params_view(params)), steps, checkpoint_interval,
+                                        C.byref(self.state_view(final_adj)), fs, C.byref(self.state_view(initial_adj)), ca, pa,
+                                        C.byref(info))
+        if check:
+            self._check(st, "ds_trajectory_vjp")
+        return initial_adj, info
+
     # ---- operators (tests)
     def masks(self):
         m = np.zeros(cell_shape(self.n), dtype=np.uint8)
@@ -385,6 +412,17 @@ class Context:
         if check:
             self._check(st, "ds_pcg_solve")
         return x, info, st
+
+    def warm_start_load(self, params: np.ndarray, output_scale: float):
+        """Load the warm-start network's FP32 parameters (state_dict order) and its output scale."""
+        flat = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        self._check(self.lib.ds_warm_start_load(self.h, flat.ctypes.data_as(C.c_void_p), flat.size, float(output_scale)),
+                    "ds_warm_start_load")
+
+    def warm_start_infer(self, b):
+        x = self.zeros_cell()
+        self._check(self.lib.ds_warm_start_infer(self.h, _ptr(b), _ptr(x)), "ds_warm_start_infer")
+        return x
 
     def profile_enable(self, sample_every: int):
         """Bracket every `sample_every`-th launch of the PCG iteration kernels with HIP events."""

@@ -1,0 +1,168 @@
+"""Warm-start network of the pressure solves: definition, data generation and training.
+
+The network is a 3-level 3-D U-Net (16/32/64 channels, 3x3x3 convolutions with ReLU, 2x2x2 max
+pooling, 2x2x2 stride-2 transposed convolutions, a 1x1x1 head; 339,681 parameters). Input: the
+right-hand side b divided by s = rms(b) over the unknowns, and the unknown indicator. Output:
+the guess x0 = head * s * output_scale (0 off the unknowns).
+
+Inference inside the solver runs in the HIP library (csrc/unet.hip, BF16 MFMA implicit-GEMM
+convolutions); this module is the training harness. It trains in PyTorch on (b, p) pairs that the
+solver itself produces: every forward step of jittered plume runs gives the converged pressure p
+(to 1e-8) and b = A p. The parameters are saved in PyTorch state_dict order, flattened to FP32,
+which is the order ds_warm_start_load expects.
+"""
+from __future__ import annotations
+
+import json
+import os
+import time
+
+import numpy as np
+
+from . import abi, scenes
+
+LAYERS = ("enc1a", "enc1b", "enc2a", "enc2b", "bota", "botb", "up2", "dec2a", "dec2b", "up1", "dec1a", "dec1b", "head")
+
+
+def build_unet():
+    import torch.nn as nn
+
+    class UNet(nn.Module):
+        def __init__(self):
+            super().__init__()
+            conv = lambda ci, co: nn.Conv3d(ci, co, 3, padding=1)
+            self.enc1a, self.enc1b = conv(2, 16), conv(16, 16)
+            self.enc2a, self.enc2b = conv(16, 32), conv(32, 32)
+            self.bota, self.botb = conv(32, 64), conv(64, 64)
+            self.up2 = nn.ConvTranspose3d(64, 32, 2, stride=2)
+            self.dec2a, self.dec2b = conv(64, 32), conv(32, 32)
+            self.up1 = nn.ConvTranspose3d(32, 16, 2, stride=2)
+            self.dec1a, self.dec1b = conv(32, 16), conv(16, 16)
+            self.head = nn.Conv3d(16, 1, 1)
+
+        def forward(self, x):
+            import torch
+            import torch.nn.functional as F
+
+            r = F.relu
+            s1 = r(self.enc1b(r(self.enc1a(x))))
+            s2 = r(self.enc2b(r(self.enc2a(F.max_pool3d(s1, 2)))))
+            c = r(self.botb(r(self.bota(F.max_pool3d(s2, 2)))))
+            d2 = r(self.dec2b(r(self.dec2a(torch.cat([self.up2(c), s2], 1)))))
+            d1 = r(self.dec1b(r(self.dec1a(torch.cat([self.up1(d2), s1], 1)))))
+            return self.head(d1)
+
+    return UNet()
+
+
+def flatten_parameters(model) -> np.ndarray:
+    return np.concatenate([t.detach().float().cpu().numpy().ravel() for t in model.state_dict().values()]).astype(np.float32)
+
+
+def load_parameters(model, flat: np.ndarray):
+    import torch
+
+    sd = model.state_dict()
+    off = 0
+    for k, t in sd.items():
+        n = t.numel()
+        sd[k] = torch.from_numpy(np.asarray(flat[off:off + n], dtype=np.float32).reshape(t.shape))
+        off += n
+    assert off == flat.size, "parameter count mismatch"
+    model.load_state_dict(sd)
+
+
+def network_input(b, unknown):
+    """(b / s, unknown) as a (1, 2, nz, ny, nx) tensor, and s = rms of b over the unknowns."""
+    import torch
+
+    nu = unknown.sum().clamp(min=1)
+    s = torch.sqrt((b.double() ** 2).sum() / nu).clamp(min=1e-30)
+    x = torch.stack([torch.where(unknown, b / s.float(), torch.zeros_like(b)), unknown.float()], 0)[None]
+    return x, s
+
+
+def save(path: str, flat: np.ndarray, output_scale: float, meta: dict):
+    np.savez(path, params=flat, output_scale=np.float64(output_scale), meta=json.dumps(meta))
+
+
+def load(path: str):
+    d = np.load(path, allow_pickle=False)
+    return d["params"], float(d["output_scale"]), json.loads(str(d["meta"]))
+
+
+def generate_pairs(n: int, runs: int, steps: int, seed: int = 100, jitter_cells: float = 8.0, tolerance: float = 1e-8,
+                   device: int = 0, log=print):
+    """(b, p) pairs from forward steps of jittered plumes at n^3, kept on the GPU."""
+    import torch
+
+    from .solver import Context
+
+    rng = np.random.default_rng(seed)
+    bs, ps = [], []
+    unknown = None
+    for r in range(runs):
+        off = rng.uniform(-jitter_cells, jitter_cells, 3) / n
+        centre = (0.5 + off[0], 0.125 + abs(off[1]) * 0.5, 0.5 + off[2])
+        sc = scenes.plume(n, noise_seed=seed + 1 + r, source_centre=centre, with_target=False)
+        ctx = Context(sc.config(tolerance=tolerance), abi.MODE_FORWARD, device)
+        state = ctx.state_from_host(scenes.initial_state(sc))
+        ctl = ctx.control_from_host(scenes.control(sc))
+        params = ctx.allocate_parameters(sc.params)
+        cache = ctx.allocate_step_cache(pressure=True)
+        nxt = ctx.allocate_state()
+        if unknown is None:
+            unknown = torch.from_numpy(ctx.masks()[1] > 0).to(ctx.device)
+        its = 0
+        for _ in range(steps):
+            ctx.forward_step(state, ctl, params, next_state=nxt, cache=cache)
+            ctx.synchronize()
+            p = (cache["pressure"][0].double() + cache["pressure"][1].double()).float()
+            b = ctx.apply_matrix(p)
+            ctx.synchronize()
+            bs.append(b.clone())
+            ps.append(p.clone())
+            state, nxt = nxt, state
+        ctx.close()
+        log(f"run {r + 1}/{runs}: {len(bs)} pairs")
+    return torch.stack(bs), torch.stack(ps), unknown
+
+
+def train(bs, ps, unknown, epochs: int = 20, batch: int = 2, lr: float = 1e-3, seed: int = 0, time_budget: float = 600.0, log=print):
+    """Adam on the relative L2 error of the guess in the A-norm proxy ||x0 - p|| / ||p||."""
+    import torch
+
+    torch.manual_seed(seed)
+    model = build_unet().to(bs.device)
+    # output scale: p / s is O(output_scale)
+    with torch.no_grad():
+        ratios = []
+        for i in range(0, len(bs), max(1, len(bs) // 16)):
+            _, s = network_input(bs[i], unknown)
+            ratios.append(float(ps[i].std() / s))
+        output_scale = float(np.median(ratios)) or 1.0
+    opt = torch.optim.Adam(model.parameters(), lr=lr)
+    n = len(bs)
+    t0 = time.time()
+    step = 0
+    for ep in range(epochs):
+        perm = torch.randperm(n, device=bs.device)
+        total = 0.0
+        for i in range(0, n, batch):
+            idx = perm[i:i + batch]
+            xs, ss = zip(*[network_input(bs[j], unknown) for j in idx.tolist()])
+            x = torch.cat(xs)
+            target = torch.stack([ps[j] / (float(s) * output_scale) for j, s in zip(idx.tolist(), ss)])[:, None]
+            mask = unknown[None, None].float()
+            out = model(x) * mask
+            err = ((out - target * mask) ** 2).flatten(1).sum(1) / (target ** 2).flatten(1).sum(1).clamp(min=1e-30)
+            loss = err.mean()
+            opt.zero_grad()
+            loss.backward()
+            opt.step()
+            total += float(loss) * len(idx)
+            step += 1
+        log(f"epoch {ep + 1}: relative squared error {total / n:.4f} ({time.time() - t0:.0f} s)")
+        if time.time() - t0 > time_budget:
+            break
+    return model, output_scale

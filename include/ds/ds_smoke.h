@@ -80,6 +80,11 @@ typedef struct ds_collider {
  * (operators.cu:885-902,1004-1013), kept as a reference mode. */
 enum { DS_PROJECTION_PCG = 0, DS_PROJECTION_RBGS = 1 };
 
+/* PCG recurrence: single-reduction (Chronopoulos-Gear: one fused kernel and one global reduction
+ * per iteration; the default) or the two-reduction textbook form (two kernels per iteration).
+ * Both iterate the same Jacobi-preconditioned CG; they differ only in rounding. */
+enum { DS_CG_SINGLE_REDUCTION = 0, DS_CG_TWO_REDUCTION = 1 };
+
 /* Initial guess of the forward and adjoint pressure solves. */
 enum { DS_WARM_START_NONE = 0, DS_WARM_START_PREVIOUS = 1, DS_WARM_START_CNN = 2 };
 
@@ -101,6 +106,7 @@ typedef struct ds_config {
     double cg_tolerance;                    /* ||b - A x||_2 <= tol * ||b||_2, FP64 norms; 0 selects 1e-6 */
     uint32_t warm_start;                    /* DS_WARM_START_* */
     uint32_t residual_replacement_interval; /* true-residual refresh period, 0 selects 50 */
+    uint32_t cg_variant;                    /* DS_CG_* recurrence of the PCG iteration */
 } ds_config;
 
 /* ExecutionMode (model.ixx:10-13). */
@@ -276,6 +282,18 @@ ds_status ds_pcg_solve(ds_context* c, const float* b, float* x, const float* x0,
 /* y = A x on unknowns (0 elsewhere); for tests. */
 ds_status ds_apply_pressure_matrix(ds_context* c, const float* x, float* y);
 
+/* ---- warm-start network (DS_WARM_START_CNN; no counterpart in the reference, whose projection
+ * starts every solve from p = 0, operators.cu:1005) ------------------------------------------ */
+/* Number of FP32 parameters of the network (a 3-level 3-D U-Net, 16/32/64 channels). */
+uint64_t ds_warm_start_parameter_count(void);
+/* Loads the network's parameters (PyTorch state_dict order of ds_smoke/warm_start.py's UNet:
+ * per layer the weight, then the bias) and the scale of its output. Needs nx % 64 == 0,
+ * ny % 16 == 0, nz % 4 == 0. With DS_WARM_START_CNN every solve without an explicit guess starts
+ * from the network's guess; the guess is rejected (x0 = 0) when it is worse than 0. */
+ds_status ds_warm_start_load(ds_context* c, const float* params, uint64_t count, double output_scale);
+/* The network's guess x for A x = b (device cell fields; b is 0 off the unknowns). */
+ds_status ds_warm_start_infer(ds_context* c, const float* b, float* x);
+
 /* ---- kernel timing (no counterpart in the reference, which times whole steps with
  * std::chrono, examples/smoke/forward/simulation.cpp:91,107-119) -------------------------- */
 /* Brackets every `sample_every`-th launch of the two PCG iteration kernels with HIP events on
@@ -324,6 +342,18 @@ ds_status ds_loss_gradient(ds_context* c, const ds_state* initial, const ds_cont
                            ds_state* final_state, ds_state_adjoint* initial_adj,
                            ds_control_adjoint* control_adj, const ds_parameter_adjoint* param_adj,
                            ds_trajectory_info* info);
+
+/* Reverse sweep for a given adjoint of the final state (dL/d state_T), as solver::vjp
+ * (solver.ixx:115-150) with a trajectory adjoint that is zero except at the final state:
+ * recomputes the forward pass with a checkpoint every `checkpoint_interval` steps and
+ * accumulates (+=) dL/d(initial state) into initial_adj, the control adjoints (as in
+ * ds_loss_gradient) and the parameter adjoints. final_state (may be NULL) receives state_T. */
+ds_status ds_trajectory_vjp(ds_context* c, const ds_state* initial, const ds_control* controls,
+                            uint32_t control_count, const ds_parameters* params, uint32_t steps,
+                            uint32_t checkpoint_interval, const ds_state_adjoint* final_adj,
+                            ds_state* final_state, ds_state_adjoint* initial_adj,
+                            ds_control_adjoint* control_adj, const ds_parameter_adjoint* param_adj,
+                            ds_trajectory_info* info);
 
 #ifdef __cplusplus
 }
