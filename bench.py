@@ -66,6 +66,24 @@ def cpu_baseline(grid, sim_steps, k, sample_steps):
     }
 
 
+def calibrate_copy(mib):
+    """Plain device copies of a known size: FETCH_SIZE / WRITE_SIZE of these launches give the
+    counter-to-bytes factor that the PCG kernels' counters are scaled by."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(10):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 10
+    print(json.dumps({"copy_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4,
+                      "ms_per_copy": dt * 1e3, "gbs": 2 * n * 4 / dt / 1e9}))
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch from the committed rocprofv3 counter summary, when present."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_plume128_traffic.json")
@@ -90,7 +108,12 @@ def main():
                     help="initial guess of each pressure solve: zero, or the previous solve's solution")
     ap.add_argument("--cg-variant", choices=["single", "two"], default="single",
                     help="PCG recurrence: single-reduction (one fused kernel per iteration) or two-reduction")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run 10 device-to-device copies of MIB MiB (the reference for HBM counter scaling)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the hot path is single-GPU here); got --gpus {args.gpus}", file=sys.stderr)
         sys.exit(2)

@@ -70,7 +70,12 @@ constexpr uint32_t kPcgThreads = 1024;
 constexpr uint32_t kPcgUnroll = 4;
 constexpr uint32_t kPcgMaxBlocks = 1024;
 
-enum Partial : int { kPartBB = 0, kPartRR = 1, kPartRZ = 2, kPartPQ = 3, kPartCount = 4 };
+enum Partial : int {
+    kPartBB = 0, kPartRR = 1, kPartRZ = 2, kPartPQ = 3,
+    // single-reduction form only: <w_{i+1}, p_i>, <s_i, u_{i+1}>, <s_i, p_i>
+    kPartWP = 4, kPartSU = 5, kPartSP = 6,
+    kPartCount = 8
+};
 
 // Device status of one PCG solve; written by block 0 of the kernel that decides.
 struct PcgStatus {

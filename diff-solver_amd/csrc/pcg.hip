@@ -324,10 +324,10 @@ PcgStatus read_status(ds_context* c) {
 
 template <int Mode>
 void launch_fused(ds_context* c, uint32_t per, const FusedVecs& v, const double* in_parts, const double* bb_parts,
-                  double* out_parts, double tol2, uint32_t k, uint32_t check_only, uint32_t respect_stop) {
+                  double* out_parts, double tol2, uint32_t k, uint32_t check_only, uint32_t respect_stop, uint32_t with_ps = 0) {
     PcgWork& w = c->pcg;
     pcg_fused_kernel<Mode><<<w.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, w.tiling, c->code, v, in_parts, bb_parts, out_parts,
-                                                                    w.hist, w.status, tol2, k, check_only, respect_stop);
+                                                                    w.hist, w.status, tol2, k, check_only, respect_stop, with_ps);
 }
 
 // Single-reduction iteration from the state left by launch_setup (r in set 0, BB/RR/RZ partials
@@ -352,8 +352,11 @@ ds_solve_info pcg_iterate_fused(ds_context* c, float* xh, float* xl, double tol,
         v.xl = xl;
         return v;
     };
-    auto prime = [&](int set, uint32_t respect_stop) {
-        launch_fused<kFusedPrime>(c, per, vecs(set, true), nullptr, nullptr, parts[set], tol2, 0u, 0u, respect_stop);
+    // with_ps: a previous direction exists (k > 0), so the denominator products are formed too
+    auto prime = [&](int set, uint32_t respect_stop, bool with_ps) {
+        FusedVecs v = vecs(set, true);
+        v.s = w.sv[set];   // s_{i-1}: the replaced s of this set
+        launch_fused<kFusedPrime>(c, per, v, nullptr, nullptr, parts[set], tol2, 0u, 0u, respect_stop, with_ps ? 1u : 0u);
     };
     auto replace = [&](int set, bool with_s, uint32_t respect_stop) {
         pcg_fused_replace_kernel<<<w.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, c->code, w.b, xh, xl, w.p[0], rv[set], w.sv[set],
@@ -363,7 +366,7 @@ ds_solve_info pcg_iterate_fused(ds_context* c, float* xh, float* xl, double tol,
     info.warm_started = warm ? 1u : 0u;
     info.guess_rejected = rejected ? 1u : 0u;
     int set = 0;
-    prime(0, 0u);
+    prime(0, 0u, false);
     check_launch("pcg_fused_kernel (prime)");
     uint32_t k = 0;
     bool first = true;
@@ -397,7 +400,7 @@ ds_solve_info pcg_iterate_fused(ds_context* c, float* xh, float* xl, double tol,
                 set ^= 1;
                 if ((k + 1u) % R == 0u && k + 1u < max_it) {
                     replace(set, true, 1u);
-                    prime(set, 1u);
+                    prime(set, 1u, true);
                 }
             }
             check_launch("pcg_fused_kernel");
@@ -441,7 +444,7 @@ ds_solve_info pcg_iterate_fused(ds_context* c, float* xh, float* xl, double tol,
         if (ok || !finite || !st.stop || done >= max_it || ++restarts > 8) break;
         // the recursive residual met the tolerance but the true one does not: continue from the
         // true residual (r and s replaced above) with the same search direction
-        prime(set, 0u);
+        prime(set, 0u, done > 0);
         pcg_clear_stop_kernel<<<1, 64, 0, c->stream>>>(w.status);
         check_launch("pcg_clear_stop_kernel");
     }

@@ -6,10 +6,16 @@
 // gamma = <r, u> and delta = <w, u>, come from the same vectors and need one global reduction:
 //
 //   beta_i  = gamma_i / gamma_{i-1}                      (0 for i = 0)
-//   alpha_i = gamma_i / (delta_i - beta_i gamma_i / alpha_{i-1})   (gamma_0 / delta_0 for i = 0)
+//   alpha_i = gamma_i / <s_i, p_i>                        (gamma_0 / delta_0 for i = 0)
 //   p_i = u_i + beta_i p_{i-1};   s_i = w_i + beta_i s_{i-1}
 //   x += alpha_i p_i;   r_{i+1} = r_i - alpha_i s_i;   u_{i+1} = D^-1 r_{i+1};   w_{i+1} = A u_{i+1}
-//   partials of gamma_{i+1}, delta_{i+1} and rho_{i+1} = ||r_{i+1}||^2
+//   partials of gamma_{i+1}, delta_{i+1}, rho_{i+1} = ||r_{i+1}||^2 and of the three products
+//   <w_{i+1}, p_i>, <s_i, u_{i+1}>, <s_i, p_i> that expand the next denominator:
+//   <s_{i+1}, p_{i+1}> = delta_{i+1} + beta (<w_{i+1}, p_i> + <s_i, u_{i+1}>) + beta^2 <s_i, p_i>.
+// Chronopoulos & Gear write the denominator as delta_i - beta_i gamma_i / alpha_{i-1}, which holds
+// only in exact arithmetic; with FP32 vectors it let the recursive residual run ahead of the true
+// one (solves that met 1e-6 recursively failed the true-residual check and restarted). The
+// expanded form is the product of the vectors the update actually uses, as in the textbook form.
 //
 // In exact arithmetic the iterates equal those of the two-reduction form. One kernel does all of
 // iteration i: every quantity is pointwise except w_{i+1} = A u_{i+1}, and u_{i+1} at a neighbour
@@ -51,9 +57,9 @@ struct FHalo {
     float r, w, s;
     uint8_t d;   // kept as loaded: a widening right after the load would wait for it
 };
-// Own-cell values only the owning thread needs.
+// Own-cell values only the owning thread needs (s: the priming pass's s_{i-1}).
 struct FOwn {
-    float p, h, l;
+    float p, h, l, s;
 };
 struct FPlane {
     FHalo c, x1, y2;
@@ -92,16 +98,17 @@ __device__ __forceinline__ float fused_u(const FHalo& h, float alpha, float beta
 }
 
 // Iteration k (or the priming pass). in_parts / out_parts: partial sets (kPartCount * kPcgMaxBlocks
-// doubles each, slots RR = rho, RZ = gamma, PQ = delta; BB read from bb_parts on the first launch
-// of a solve). hist[2k] = gamma_k, hist[2k+1] = alpha_k. check_only: convergence test only.
+// doubles each, slots RR = rho, RZ = gamma, PQ = delta, WP, SU, SP; BB read from bb_parts on the
+// first launch of a solve). hist[2k] = gamma_k, hist[2k+1] = alpha_k. check_only: convergence
+// test only. with_ps (priming pass): p_{i-1} and s_{i-1} exist, so form WP, SU, SP too.
 template <int Mode>
 __global__ __launch_bounds__(1024) void pcg_fused_kernel(Geo g, uint32_t per, PlaneTiling pt, const uint8_t* __restrict__ code,
                                                          FusedVecs v, const double* __restrict__ in_parts,
                                                          const double* __restrict__ bb_parts, double* __restrict__ out_parts,
                                                          double* __restrict__ hist, PcgStatus* status, double tol2, uint32_t k,
-                                                         uint32_t check_only, uint32_t respect_stop) {
+                                                         uint32_t check_only, uint32_t respect_stop, uint32_t with_ps) {
     __shared__ float tile[2][kTY + 2][kTX + 2];
-    __shared__ double lds[4 * kMaxWaves];
+    __shared__ double lds[7 * kMaxWaves];
     if ((Mode != kFusedPrime || respect_stop) && status->stop) return;
     const uint32_t lb = xcd_block();
     int x0, y0, z0, z1;
@@ -120,12 +127,12 @@ __global__ __launch_bounds__(1024) void pcg_fused_kernel(Geo g, uint32_t per, Pl
     const bool has_part = threadIdx.x < gridDim.x;
     const uint32_t tp = has_part ? threadIdx.x : 0u;
     const double* bbp = bb_parts ? bb_parts : in_parts;
-    double part[4]{0.0, 0.0, 0.0, 0.0};
+    constexpr int kSlots[7]{kPartRR, kPartRZ, kPartPQ, kPartWP, kPartSU, kPartSP, kPartBB};
+    double part[7]{};
     if (Mode != kFusedPrime) {
-        part[0] = in_parts[kPartRR * kPcgMaxBlocks + tp];
-        part[1] = in_parts[kPartRZ * kPcgMaxBlocks + tp];
-        part[2] = in_parts[kPartPQ * kPcgMaxBlocks + tp];
-        part[3] = bbp[kPartBB * kPcgMaxBlocks + tp];
+#pragma unroll
+        for (int q = 0; q < 6; ++q) part[q] = in_parts[kSlots[q] * kPcgMaxBlocks + tp];
+        part[6] = bbp[kPartBB * kPcgMaxBlocks + tp];
     }
 
     // loads of plane zz: the own column up to the top halo plane z1 (planes beyond read the pad);
@@ -140,13 +147,16 @@ __global__ __launch_bounds__(1024) void pcg_fused_kernel(Geo g, uint32_t per, Pl
             P.y2 = FHalo{0.0f, 0.0f, 0.0f, 0u};
         const int64_t j = zz <= z1 ? wrapped_index(g, per, x, y, zz) : -1;
         P.c = fused_load_halo<Mode>(halo_index(g, j), v, code);
+        const uint32_t jo = j >= 0 ? (uint32_t)j : 0u;
         if (kUpdate) {
-            const uint32_t jo = j >= 0 ? (uint32_t)j : 0u;
             P.o.h = v.xh[jo];
             P.o.l = v.xl[jo];
             P.o.p = Mode == kFusedNext ? v.p[jo] : 0.0f;
+            P.o.s = 0.0f;
+        } else if (with_ps) {   // block-uniform
+            P.o = FOwn{v.p[jo], 0.0f, 0.0f, v.s[jo]};
         } else {
-            P.o = FOwn{0.0f, 0.0f, 0.0f};
+            P.o = FOwn{0.0f, 0.0f, 0.0f, 0.0f};
         }
     };
 
@@ -161,12 +171,12 @@ __global__ __launch_bounds__(1024) void pcg_fused_kernel(Geo g, uint32_t per, Pl
 
     float alpha = 0.0f, beta = 0.0f;
     if (Mode != kFusedPrime) {
-        double red[4];
+        double red[7];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) red[q] = has_part ? part[q] : 0.0;
-        block_sum<4>(red, lds);
+        for (int q = 0; q < 7; ++q) red[q] = has_part ? part[q] : 0.0;
+        block_sum<7>(red, lds);
         const double rho = red[0], gamma = red[1], delta = red[2];
-        const double bb = bb_parts ? red[3] : status->bb;
+        const double bb = bb_parts ? red[6] : status->bb;
         if (!(rho > tol2 * bb) || check_only) {   // converged (NaN also stops; the true residual reports it)
             if (lb == 0 && threadIdx.x == 0) {
                 if (bb_parts) status->bb = bb;
@@ -179,7 +189,7 @@ __global__ __launch_bounds__(1024) void pcg_fused_kernel(Geo g, uint32_t per, Pl
         double b64 = 0.0, a64;
         if (Mode == kFusedNext) {
             b64 = gamma / hist[2 * (k - 1)];
-            a64 = gamma / (delta - b64 * gamma / hist[2 * (k - 1) + 1]);
+            a64 = gamma / (delta + b64 * (red[3] + red[4]) + b64 * b64 * red[5]);
         } else {
             a64 = gamma / delta;
         }
@@ -192,11 +202,13 @@ __global__ __launch_bounds__(1024) void pcg_fused_kernel(Geo g, uint32_t per, Pl
         beta = (float)b64;
     }
 
-    double rho = 0.0, gam = 0.0, del = 0.0;
+    double rho = 0.0, gam = 0.0, del = 0.0, wp = 0.0, su = 0.0, sp = 0.0;
     if (z0 < z1) {
-        // pointwise part of plane zz for the own cell: updates, stores, gamma/rho partials; returns u_{i+1}
-        auto own_update = [&](const FPlane& P, int zz) -> float {
+        // pointwise part of plane zz for the own cell: updates, stores, partials; returns u_{i+1}
+        // and leaves p_i of the cell in pz (for <w_{i+1}, p_i> at the plane's stencil step)
+        auto own_update = [&](const FPlane& P, int zz, float& pz) -> float {
             const float u = fused_u<Mode>(P.c, alpha, beta);
+            pz = 0.0f;
             if (own && zz < z1) {
                 const uint32_t i = cidx(g, x, y, zz);
                 const float rn = fused_r<Mode>(P.c, alpha, beta);
@@ -210,6 +222,14 @@ __global__ __launch_bounds__(1024) void pcg_fused_kernel(Geo g, uint32_t per, Pl
                     v.xh[i] = h;
                     v.xl[i] = l;
                     v.s_out[i] = si;
+                    pz = pi;
+                    su += (double)si * u;
+                    sp += (double)si * pi;
+                }
+                if (!kUpdate && P.c.d != 0u) {   // priming: the existing p_{i-1}, s_{i-1}
+                    pz = P.o.p;
+                    su += (double)P.o.s * u;
+                    sp += (double)P.o.s * P.o.p;
                 }
                 if (kUpdate) v.r_out[i] = P.c.d != 0u ? rn : 0.0f;
                 rho += (double)rn * rn;
@@ -222,7 +242,8 @@ __global__ __launch_bounds__(1024) void pcg_fused_kernel(Geo g, uint32_t per, Pl
             if (sl.has2) tile[b][sl.t2y][lx + 1] = fused_u<Mode>(P.y2, alpha, beta);
         };
         float below = fused_u<Mode>(below_raw, alpha, beta);
-        float centre = own_update(cur, z0);
+        float pc = 0.0f;
+        float centre = own_update(cur, z0, pc);
         uint8_t d = cur.c.d;
         tile[0][ly + 1][lx + 1] = centre;
         stage(cur, 0);
@@ -232,7 +253,8 @@ __global__ __launch_bounds__(1024) void pcg_fused_kernel(Geo g, uint32_t per, Pl
         // z+3. A step with z >= z1 (the second step of an odd-length chunk) only loads and waits,
         // so that both steps of the unrolled loop run unconditionally.
         auto step = [&](FPlane& S, int z) {
-            const float next = own_update(S, z + 1);
+            float pn = 0.0f;
+            const float next = own_update(S, z + 1, pn);
             if (z + 1 < z1) {
                 tile[buf ^ 1][ly + 1][lx + 1] = next;
                 stage(S, buf ^ 1);
@@ -246,12 +268,14 @@ __global__ __launch_bounds__(1024) void pcg_fused_kernel(Geo g, uint32_t per, Pl
                           below + next);
                 v.w_out[i] = wv;
                 del += (double)wv * centre;
+                wp += (double)wv * pc;
             }
             const uint8_t dn = S.c.d;
             load_plane(z + 3, S);
             __syncthreads();
             below = centre;
             centre = next;
+            pc = pn;
             d = dn;
             buf ^= 1;
         };
@@ -260,12 +284,15 @@ __global__ __launch_bounds__(1024) void pcg_fused_kernel(Geo g, uint32_t per, Pl
             step(slot_b, z + 1);
         }
     }
-    double red[3]{rho, gam, del};
-    block_sum<3>(red, lds);
+    double red[6]{rho, gam, del, wp, su, sp};
+    block_sum<6>(red, lds);
     if (threadIdx.x == 0) {
         out_parts[kPartRR * kPcgMaxBlocks + lb] = red[0];
         out_parts[kPartRZ * kPcgMaxBlocks + lb] = red[1];
         out_parts[kPartPQ * kPcgMaxBlocks + lb] = red[2];
+        out_parts[kPartWP * kPcgMaxBlocks + lb] = red[3];
+        out_parts[kPartSU * kPcgMaxBlocks + lb] = red[4];
+        out_parts[kPartSP * kPcgMaxBlocks + lb] = red[5];
     }
 }
 

@@ -105,7 +105,7 @@ def generate_pairs(n: int, runs: int, steps: int, seed: int = 100, jitter_cells:
         off = rng.uniform(-jitter_cells, jitter_cells, 3) / n
         centre = (0.5 + off[0], 0.125 + abs(off[1]) * 0.5, 0.5 + off[2])
         sc = scenes.plume(n, noise_seed=seed + 1 + r, source_centre=centre, with_target=False)
-        ctx = Context(sc.config(tolerance=tolerance), abi.MODE_FORWARD, device)
+        ctx = Context(sc.config(tolerance=tolerance, max_iterations=6000), abi.MODE_FORWARD, device)
         state = ctx.state_from_host(scenes.initial_state(sc))
         ctl = ctx.control_from_host(scenes.control(sc))
         params = ctx.allocate_parameters(sc.params)
@@ -162,7 +162,11 @@ def train(bs, ps, unknown, epochs: int = 20, batch: int = 2, lr: float = 1e-3, s
             opt.step()
             total += float(loss) * len(idx)
             step += 1
-        log(f"epoch {ep + 1}: relative squared error {total / n:.4f} ({time.time() - t0:.0f} s)")
+            if step % 20 == 0:
+                log(f"step {step}: batch relative squared error {float(loss):.4f} ({time.time() - t0:.0f} s)")
+            if time.time() - t0 > time_budget:
+                break
+        log(f"epoch {ep + 1}: relative squared error {total / max(1, min(n, i + batch)):.4f} ({time.time() - t0:.0f} s)")
         if time.time() - t0 > time_budget:
             break
     return model, output_scale
