@@ -18,24 +18,32 @@ namespace ds {
 
 namespace {
 
+// Sink of the three velocity-adjoint components: plain global atomics.
+struct VelSink {
+    F3 va;
+    template <int B>
+    __device__ __forceinline__ void add(int, int, int, uint32_t i, float value) const { atomicAdd(va.p[B] + i, value); }
+};
+
 // Reverse of trace_rk2 + final sample (operators.cu:422-438 / 544-560): given the adjoint of the
 // trace end point (pbx, pby, pbz) = -deriv * dt * grad(sample) * out_adj, scatter through the
 // midpoint and start samples of the velocity.
-__device__ __forceinline__ void trace_reverse(const Geo& g, const VBound& b, const Vel3& u, float* const va[3], float sx, float sy,
+template <typename Sink>
+__device__ __forceinline__ void trace_reverse(const Geo& g, const VBound& b, const Vel3& u, const Sink& sink, float sx, float sy,
                                               float sz, const TraceOut& t, float v1x, float v1y, float v1z) {
     const Sample m0 = sample_face<0, true>(u.u[0], t.mx, t.my, t.mz, g, b);
     const Sample m1 = sample_face<1, true>(u.u[1], t.mx, t.my, t.mz, g, b);
     const Sample m2 = sample_face<2, true>(u.u[2], t.mx, t.my, t.mz, g, b);
-    scatter_face<0>(va[0], t.mx, t.my, t.mz, v1x, g, b);
-    scatter_face<1>(va[1], t.mx, t.my, t.mz, v1y, g, b);
-    scatter_face<2>(va[2], t.mx, t.my, t.mz, v1z, g, b);
+    scatter_face_to<0>(sink, t.mx, t.my, t.mz, v1x, g, b);
+    scatter_face_to<1>(sink, t.mx, t.my, t.mz, v1y, g, b);
+    scatter_face_to<2>(sink, t.mx, t.my, t.mz, v1z, g, b);
     const float mx = m0.gx * v1x + m1.gx * v1y + m2.gx * v1z;
     const float my = m0.gy * v1x + m1.gy * v1y + m2.gy * v1z;
     const float mz = m0.gz * v1x + m1.gz * v1y + m2.gz * v1z;
     const float s = -0.5f * g.dt;
-    scatter_face<0>(va[0], sx, sy, sz, s * mx, g, b);
-    scatter_face<1>(va[1], sx, sy, sz, s * my, g, b);
-    scatter_face<2>(va[2], sx, sy, sz, s * mz, g, b);
+    scatter_face_to<0>(sink, sx, sy, sz, s * mx, g, b);
+    scatter_face_to<1>(sink, sx, sy, sz, s * my, g, b);
+    scatter_face_to<2>(sink, sx, sy, sz, s * mz, g, b);
 }
 
 // advect_scalar_vjp_kernel (operators.cu:537-561) for NF fields that share one trace: the
@@ -70,30 +78,83 @@ __global__ __launch_bounds__(256) void advect_scalar_vjp_kernel(Geo g, const VBo
         gz += s1.gz * a1;
     }
     const float v1x = -(t.dx * g.dt) * gx, v1y = -(t.dy * g.dt) * gy, v1z = -(t.dz * g.dt) * gz;
-    trace_reverse(g, vb, u, va.p, sx, sy, sz, t, v1x, v1y, v1z);
+    trace_reverse(g, vb, u, VelSink{va}, sx, sy, sz, t, v1x, v1y, v1z);
 }
 
-// advect_velocity_vjp_kernel, operators.cu:415-439
-template <bool HasSolids>
+// advect_velocity_vjp_kernel (operators.cu:415-439) for the faces of axis A, with its scatters
+// privatised per block: a block owns a kVjpBX x kVjpBY x kVjpBZ brick of A-faces, whose traces
+// move a few cells at most, so their 56 scatters per face land in a small box around the brick.
+// Scatters inside the box (brick plus a halo of kVjpHalo faces, each component) are LDS atomics;
+// those outside it (long traces, periodic wraps) stay global atomics. The box is flushed once.
+constexpr int kVjpBX = 32, kVjpBY = 8, kVjpBZ = 4, kVjpHalo = 2;
+constexpr int kWinX = kVjpBX + 2 * kVjpHalo + 1, kWinY = kVjpBY + 2 * kVjpHalo + 1, kWinZ = kVjpBZ + 2 * kVjpHalo + 1;
+constexpr int kWin = kWinX * kWinY * kWinZ;   // per component
+
+struct WindowSink {
+    float* win;        // LDS, 3 * kWin
+    int ox, oy, oz;    // face coordinates of window element (0, 0, 0)
+    F3 va;
+    template <int B>
+    __device__ __forceinline__ void add(int x, int y, int z, uint32_t i, float value) const {
+        const unsigned lx = (unsigned)(x - ox), ly = (unsigned)(y - oy), lz = (unsigned)(z - oz);
+        // workgroup scope for the LDS add: it keeps the two atomics distinct (merged into one
+        // atomic on a selected pointer they would both become flat atomics)
+        if (lx < (unsigned)kWinX && ly < (unsigned)kWinY && lz < (unsigned)kWinZ)
+            __hip_atomic_fetch_add(win + B * kWin + ((int)lz * kWinY + (int)ly) * kWinX + (int)lx, value, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            atomicAdd(va.p[B] + i, value);
+    }
+};
+
+__device__ __forceinline__ uint32_t vjp_bricks(const Geo& g, int A, int& nbx, int& nby) {
+    nbx = (ext(g, A, 0) + kVjpBX - 1) / kVjpBX;
+    nby = (ext(g, A, 1) + kVjpBY - 1) / kVjpBY;
+    return (uint32_t)nbx * nby * ((ext(g, A, 2) + kVjpBZ - 1) / kVjpBZ);
+}
+
+template <int A, bool HasSolids>
 __global__ __launch_bounds__(256) void advect_velocity_vjp_kernel(Geo g, const VBound* __restrict__ bp, const uint8_t* __restrict__ mask,
-                                                                  Vel3 u, CF3 oa, F3 va) {
+                                                                  Vel3 u, const float* __restrict__ oa, F3 va) {
+    __shared__ float win[3 * kWin];
     const VBound& b = *bp;
-    const uint32_t fi = xcd_block() * blockDim.x + threadIdx.x;
-    if (fi >= g.foff[3]) return;
-    for_face(g, fi, [&]<int A>(uint32_t local) {
-        const float a = oa.p[A][local];
-        if (a == 0.0f) return;
-        int x, y, z;
-        decode_face_axis<A>(g, local, x, y, z);
+    int nbx, nby;
+    const uint32_t bricks = vjp_bricks(g, A, nbx, nby);
+    const uint32_t lb = xcd_block();
+    if (lb >= bricks) return;   // padding blocks of the XCD-rounded grid (whole block)
+    const int x0 = (int)(lb % (uint32_t)nbx) * kVjpBX;
+    const int y0 = (int)((lb / (uint32_t)nbx) % (uint32_t)nby) * kVjpBY;
+    const int z0 = (int)(lb / ((uint32_t)nbx * nby)) * kVjpBZ;
+    for (int i = threadIdx.x; i < 3 * kWin; i += blockDim.x) win[i] = 0.0f;
+    __syncthreads();
+    const WindowSink sink{win, x0 - kVjpHalo, y0 - kVjpHalo, z0 - kVjpHalo, va};
+    const int ex = ext(g, A, 0), ey = ext(g, A, 1), ez = ext(g, A, 2);
+    const int x = x0 + (int)(threadIdx.x % kVjpBX), y = y0 + (int)(threadIdx.x / kVjpBX);
+    for (int f = 0; f < kVjpBZ; ++f) {
+        const int z = z0 + f;
+        if (x >= ex || y >= ey || z >= ez) continue;
+        const float a = oa[fidx(g, A, x, y, z)];
+        if (a == 0.0f) continue;
         const float sx = (x + (A == 0 ? 0.0f : 0.5f)) * g.h;
         const float sy = (y + (A == 1 ? 0.0f : 0.5f)) * g.h;
         const float sz = (z + (A == 2 ? 0.0f : 0.5f)) * g.h;
         const TraceOut t = trace_rk2<HasSolids>(sx, sy, sz, u, mask, g, b);
         const Sample s = sample_face<A, true>(u.u[A], t.px, t.py, t.pz, g, b);
-        scatter_face<A>(va.p[A], t.px, t.py, t.pz, a, g, b);
+        scatter_face_to<A>(sink, t.px, t.py, t.pz, a, g, b);
         const float v1x = -(t.dx * g.dt) * (s.gx * a), v1y = -(t.dy * g.dt) * (s.gy * a), v1z = -(t.dz * g.dt) * (s.gz * a);
-        trace_reverse(g, b, u, va.p, sx, sy, sz, t, v1x, v1y, v1z);
-    });
+        trace_reverse(g, b, u, sink, sx, sy, sz, t, v1x, v1y, v1z);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 3 * kWin; i += blockDim.x) {
+        const float v = win[i];
+        if (v == 0.0f) continue;
+        const int B = i / kWin, r = i - B * kWin;
+        const int wx = x0 - kVjpHalo + r % kWinX, wy = y0 - kVjpHalo + (r / kWinX) % kWinY, wz = z0 - kVjpHalo + r / (kWinX * kWinY);
+        // only in-range faces receive window adds; the test keeps the flush in bounds regardless
+        if (wx < 0 || wy < 0 || wz < 0 || wx >= ext(g, B, 0) || wy >= ext(g, B, 1) || wz >= ext(g, B, 2)) continue;
+        float* dst = B == 0 ? va.p[0] : B == 1 ? va.p[1] : va.p[2];
+        atomicAdd(dst + fidx(g, B, wx, wy, wz), v);
+    }
 }
 
 template <int A>
@@ -372,9 +433,17 @@ void launch_advect_scalar_vjp(ds_context* c, int nf, int first_field, const floa
 
 void launch_advect_velocity_vjp(ds_context* c, CF3 u, CF3 oa, F3 va) {
     const Vel3 v{{u.p[0], u.p[1], u.p[2]}};
-    const uint32_t G = xcd_blocks_for(c->geo.foff[3]);
-    if (c->has_solids) advect_velocity_vjp_kernel<true><<<G, kBlock, 0, c->stream>>>(c->geo, c->d_vb, c->mask, v, oa, va);
-    else advect_velocity_vjp_kernel<false><<<G, kBlock, 0, c->stream>>>(c->geo, c->d_vb, c->mask, v, oa, va);
+    const Geo& g = c->geo;
+    auto launch = [&]<int A>() {
+        const int e[3]{g.nx + (A == 0), g.ny + (A == 1), g.nz + (A == 2)};   // face extents of axis A
+        const uint64_t bricks = (uint64_t)((e[0] + kVjpBX - 1) / kVjpBX) * ((e[1] + kVjpBY - 1) / kVjpBY) * ((e[2] + kVjpBZ - 1) / kVjpBZ);
+        const uint32_t G = (uint32_t)((bricks + 7) & ~7ull);
+        if (c->has_solids) advect_velocity_vjp_kernel<A, true><<<G, kBlock, 0, c->stream>>>(g, c->d_vb, c->mask, v, oa.p[A], va);
+        else advect_velocity_vjp_kernel<A, false><<<G, kBlock, 0, c->stream>>>(g, c->d_vb, c->mask, v, oa.p[A], va);
+    };
+    launch.template operator()<0>();
+    launch.template operator()<1>();
+    launch.template operator()<2>();
     check_launch("advect_velocity_vjp_kernel");
 }
 

@@ -398,14 +398,16 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
             }
         }
         {
-            // about one block per CU: zc = tiles * nz / 256 rounded up to a power of two in [4, 64]
+            // about kPcgTargetBlocks blocks: zc = tiles * nz / kPcgTargetBlocks rounded up to a power of two in [4, 64]
             PlaneTiling& t = c->pcg.tiling;
             t.tiles_x = (uint32_t)((g.nx + kPlaneTileX - 1) / kPlaneTileX);
             t.tiles_y = (uint32_t)((g.ny + kPlaneTileY - 1) / kPlaneTileY);
             const uint64_t tiles = (uint64_t)t.tiles_x * t.tiles_y;
             uint32_t zc = 4;
-            while (zc < 64 && tiles * (uint64_t)g.nz > 256ull * zc) zc *= 2;
-            while (tiles * (((uint64_t)g.nz + zc - 1) / zc) > kPcgMaxBlocks) zc *= 2;
+            // the fused kernel reads one producer partial per thread: blocks <= kPcgThreads
+            const uint64_t cap = std::min<uint64_t>(kPcgMaxBlocks, kPcgThreads);
+            while (zc < 64 && tiles * (uint64_t)g.nz > kPcgTargetBlocks * zc) zc *= 2;
+            while (tiles * (((uint64_t)g.nz + zc - 1) / zc) > cap) zc *= 2;
             t.zc = zc;
             const uint64_t G = tiles * (((uint64_t)g.nz + zc - 1) / zc);
             c->pcg.blocks = (uint32_t)((G + 7) & ~7ull);

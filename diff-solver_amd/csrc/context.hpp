@@ -66,7 +66,11 @@ inline uint32_t xcd_blocks_for(uint64_t count, uint32_t block = kBlock) { return
 constexpr uint32_t kPcgBlocks = 512;
 // PCG iteration kernels: 1024-thread blocks, each over a contiguous range of whole multiples of
 // kPcgUnroll * 1024 cells; the block count (<= kPcgMaxBlocks, a multiple of 8) is per context.
-constexpr uint32_t kPcgThreads = 1024;
+// (measured on the 128^3 plume: 512 threads / ~512 blocks, i.e. two independent 8-wave blocks per
+// CU, 29.3 us per fused iteration; 1024 / 256: 32.1 us; 512 / 256: 30.8 us; 1024 / 512: 44.3 us)
+constexpr uint32_t kPcgThreads = 512;
+// Block count the plane-marching iteration kernels aim for (the z chunk is sized to it).
+constexpr uint64_t kPcgTargetBlocks = 512;
 constexpr uint32_t kPcgUnroll = 4;
 constexpr uint32_t kPcgMaxBlocks = 1024;
 
@@ -112,7 +116,7 @@ struct Profiler {
 
 // Plane-marching tiling of the PCG iteration kernels: 64 x 16 (x, y) tiles, zc planes per block.
 constexpr int kPlaneTileX = 64;
-constexpr int kPlaneTileY = 16;
+constexpr int kPlaneTileY = (int)(kPcgThreads / 64);   // one wave per tile row
 struct PlaneTiling {
     uint32_t tiles_x, tiles_y, zc;
 };

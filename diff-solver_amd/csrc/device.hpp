@@ -264,9 +264,17 @@ __device__ __forceinline__ Sample sample_scalar(const float* __restrict__ v, flo
     return trilerp<Grad>(vals, c, g.inv_h);
 }
 
-// scatter_face (operators.cu:217-240): transpose of sample_face's value; FP32 atomics.
-template <int A>
-__device__ __forceinline__ void scatter_face(float* v, float px, float py, float pz, float adj, const Geo& g, const VBound& b) {
+// Sink of a scatter: adds value to face (x, y, z) of axis A (flat index i). The default adds to
+// the global array with an FP32 atomic.
+struct GlobalSink {
+    float* v;
+    template <int A>
+    __device__ __forceinline__ void add(int, int, int, uint32_t i, float value) const { atomicAdd(v + i, value); }
+};
+
+// scatter_face (operators.cu:217-240): transpose of sample_face's value.
+template <int A, typename Sink>
+__device__ __forceinline__ void scatter_face_to(const Sink& sink, float px, float py, float pz, float adj, const Geo& g, const VBound& b) {
     if (adj == 0.0f) return;
     const Corner8 c = corners(g, px, py, pz, A == 0 ? 0.0f : 0.5f, A == 1 ? 0.0f : 0.5f, A == 2 ? 0.0f : 0.5f);
     const int ex = ext(g, A, 0), ey = ext(g, A, 1);
@@ -278,19 +286,28 @@ __device__ __forceinline__ void scatter_face(float* v, float px, float py, float
         for (int k = 0; k < 8; ++k) {
             const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
             const float wk = w[dx][0] * w[dy][1] * w[dz][2];
-            if (wk != 0.0f) atomicAdd(v + base + dx + dy * sy + dz * sz, adj * wk);
+            if (wk != 0.0f) sink.template add<A>(c.x0 + dx, c.y0 + dy, c.z0 + dz, base + dx + dy * sy + dz * sz, adj * wk);
         }
     } else {
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
             const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
-            const int x = c.x0 + dx, y = c.y0 + dy, z = c.z0 + dz;
+            int x = c.x0 + dx, y = c.y0 + dy, z = c.z0 + dz;
             float cv;
             if (face_const<A>(g, x, y, z, b, cv)) continue;
             const float wk = w[dx][0] * w[dy][1] * w[dz][2];
-            if (wk != 0.0f) atomicAdd(v + face_mapped<A>(g, x, y, z, b), adj * wk);
+            if (wk == 0.0f) continue;
+            x = mapc(x, ext(g, A, 0), vperiodic(b, 0), g.nx);
+            y = mapc(y, ext(g, A, 1), vperiodic(b, 1), g.ny);
+            z = mapc(z, ext(g, A, 2), vperiodic(b, 2), g.nz);
+            sink.template add<A>(x, y, z, fidx(g, A, x, y, z), adj * wk);
         }
     }
+}
+
+template <int A>
+__device__ __forceinline__ void scatter_face(float* v, float px, float py, float pz, float adj, const Geo& g, const VBound& b) {
+    scatter_face_to<A>(GlobalSink{v}, px, py, pz, adj, g, b);
 }
 
 // scatter_scalar (operators.cu:196-215).

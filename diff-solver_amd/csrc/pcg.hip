@@ -123,7 +123,7 @@ __device__ __forceinline__ float cell_rhs(const Geo& g, const RhsIn& in, const u
 
 // Setup: b (computed or given), x = x0 (or 0), r = b - A x0 (FP64), partials ||b||^2, ||r||^2, <r,z>.
 template <int Kind>
-__global__ __launch_bounds__(1024) void pcg_setup_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
+__global__ __launch_bounds__(kPcgThreads) void pcg_setup_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
                                                         const uint8_t* __restrict__ mask, RhsIn in,
                                                         const float* __restrict__ fixed_term, float* __restrict__ b,
                                                         const float* x0h, const float* x0l, uint32_t x_is_guess,
@@ -179,7 +179,7 @@ __global__ __launch_bounds__(1024) void pcg_setup_kernel(Geo g, uint32_t per, co
 }
 
 // True residual r = b - A (x_hi + x_lo) in FP64; replaces the recursive residual.
-__global__ __launch_bounds__(1024) void pcg_true_residual_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
+__global__ __launch_bounds__(kPcgThreads) void pcg_true_residual_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
                                                                 const float* __restrict__ b, const float* __restrict__ xh,
                                                                 const float* __restrict__ xl, float* __restrict__ r,
                                                                 double* __restrict__ partials, PcgStatus* status,
@@ -213,7 +213,7 @@ __global__ __launch_bounds__(1024) void pcg_true_residual_kernel(Geo g, uint32_t
 
 // Residual replacement of the single-reduction form: r = b - A (x_hi + x_lo) and s = A p in FP64,
 // rounded to FP32 (with_s = 0 when no direction exists yet); partial ||r||^2 into out_parts.
-__global__ __launch_bounds__(1024) void pcg_fused_replace_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
+__global__ __launch_bounds__(kPcgThreads) void pcg_fused_replace_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
                                                                 const float* __restrict__ b, const float* __restrict__ xh,
                                                                 const float* __restrict__ xl, const float* __restrict__ p,
                                                                 float* __restrict__ r, float* __restrict__ s, uint32_t with_s,

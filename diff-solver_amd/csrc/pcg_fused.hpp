@@ -102,7 +102,7 @@ __device__ __forceinline__ float fused_u(const FHalo& h, float alpha, float beta
 // first launch of a solve). hist[2k] = gamma_k, hist[2k+1] = alpha_k. check_only: convergence
 // test only. with_ps (priming pass): p_{i-1} and s_{i-1} exist, so form WP, SU, SP too.
 template <int Mode>
-__global__ __launch_bounds__(1024) void pcg_fused_kernel(Geo g, uint32_t per, PlaneTiling pt, const uint8_t* __restrict__ code,
+__global__ __launch_bounds__(kPcgThreads) void pcg_fused_kernel(Geo g, uint32_t per, PlaneTiling pt, const uint8_t* __restrict__ code,
                                                          FusedVecs v, const double* __restrict__ in_parts,
                                                          const double* __restrict__ bb_parts, double* __restrict__ out_parts,
                                                          double* __restrict__ hist, PcgStatus* status, double tol2, uint32_t k,

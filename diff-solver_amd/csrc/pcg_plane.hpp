@@ -121,7 +121,7 @@ constexpr int kPrefetch = 4;
 // Iteration k, first kernel: convergence test on the reduced ||r||^2, then the new direction
 // p_k = z_k + beta p_{k-1} and the partial <p_k, A p_k>.
 template <bool HasOld>
-__global__ __launch_bounds__(1024) void pcg_direction_kernel(Geo g, uint32_t per, PlaneTiling pt, const uint8_t* __restrict__ code,
+__global__ __launch_bounds__(kPcgThreads) void pcg_direction_kernel(Geo g, uint32_t per, PlaneTiling pt, const uint8_t* __restrict__ code,
                                                              const float* __restrict__ r, const float* __restrict__ p_old,
                                                              float* __restrict__ p_new, double* __restrict__ partials,
                                                              double* __restrict__ hist, PcgStatus* status, double tol2, uint32_t k,
@@ -241,7 +241,7 @@ __device__ __forceinline__ RawUpd load_upd(int64_t j, bool full, const float* __
 __device__ __forceinline__ float load_p(int64_t j, const float* __restrict__ p) { return j >= 0 ? p[j] : 0.0f; }
 
 // Iteration k, second kernel: alpha = <r,z>/<p,Ap>, x += alpha p, r -= alpha A p, partials.
-__global__ __launch_bounds__(1024) void pcg_update_kernel(Geo g, uint32_t per, PlaneTiling pt, const uint8_t* __restrict__ code,
+__global__ __launch_bounds__(kPcgThreads) void pcg_update_kernel(Geo g, uint32_t per, PlaneTiling pt, const uint8_t* __restrict__ code,
                                                           const float* __restrict__ p, float* __restrict__ r, float* __restrict__ xh,
                                                           float* __restrict__ xl, double* __restrict__ partials,
                                                           double* __restrict__ hist, PcgStatus* status, uint32_t k) {

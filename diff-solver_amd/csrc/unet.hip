@@ -204,6 +204,55 @@ __global__ __launch_bounds__(256) void head_kernel(uint32_t n, const uint16_t* _
     guess[i] = code[i] != 0u ? (float)((double)v * scale[0] * output_scale) : 0.0f;
 }
 
+// A-norm-optimal scale of the guess g: c = <g, b> / <g, A g> minimises ||c g - x*||_A, so the
+// scaled guess is never worse than x0 = 0 in the norm CG minimises. Pass 1: per-block partials of
+// <g, b> and <g, A g> (A g gathered on the fly, g = 0 off the unknowns); pass 2: every block
+// reduces them in the same order and scales its share of g.
+__global__ __launch_bounds__(256) void guess_scale_partials_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
+                                                                   const float* __restrict__ x, const float* __restrict__ b,
+                                                                   double* __restrict__ partials) {
+    __shared__ double lds[3 * kMaxWaves];
+    double gb = 0.0, gag = 0.0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x) {
+        const uint32_t d = code[i];
+        if (d == 0u) continue;
+        int xx, yy, zz;
+        decode_cell(g, i, xx, yy, zz);
+        const double xi = x[i];
+        double s = 0.0;
+        const int c[3]{xx, yy, zz}, n[3]{g.nx, g.ny, g.nz};
+        for (int a = 0; a < 3; ++a)
+            for (int dir = -1; dir <= 1; dir += 2) {
+                int q[3]{c[0], c[1], c[2]};
+                q[a] += dir;
+                if (q[a] < 0 || q[a] >= n[a]) {
+                    if (!(per & (1u << a))) continue;
+                    q[a] = q[a] < 0 ? n[a] - 1 : 0;
+                }
+                s += (double)x[cidx(g, q[0], q[1], q[2])];
+            }
+        gb += xi * (double)b[i];
+        gag += xi * ((double)d * xi - s);
+    }
+    double v[2]{gb, gag};
+    block_sum<2>(v, lds);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = v[0];
+        partials[gridDim.x + blockIdx.x] = v[1];
+    }
+}
+
+__global__ __launch_bounds__(256) void guess_scale_apply_kernel(uint32_t n, const double* __restrict__ partials, uint32_t G,
+                                                                float* __restrict__ x) {
+    __shared__ double lds[3 * kMaxWaves];
+    const double* parts[2]{partials, partials + G};
+    double red[2];
+    reduce_partials<2>(parts, G, red, lds);
+    const double c = red[1] > 0.0 && std::isfinite(red[0] / red[1]) ? red[0] / red[1] : 0.0;
+    const float cf = (float)c;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) x[i] *= cf;
+}
+
 }  // namespace
 
 // Parameters in PyTorch state_dict order (weight then bias per layer; Conv3d weights [co][ci][kz][ky][kx],
@@ -236,6 +285,7 @@ struct UNet {
     uint16_t *c1 = nullptr, *c2 = nullptr;
     double* scale = nullptr;
     float* guess = nullptr;
+    double* scale_partials = nullptr;   // 2 * kPcgBlocks
 };
 
 void unet_release(UNet* u) {
@@ -318,12 +368,13 @@ void unet_load(ds_context* c, const float* params, uint64_t count, double output
         u->c2 = unet_alloc<uint16_t>(u, 64 * nc);
         u->scale = unet_alloc<double>(u, 1);
         u->guess = unet_alloc<float>(u, n);
+        u->scale_partials = unet_alloc<double>(u, 2 * (size_t)kPcgBlocks);
     }
     DS_HIP(hipStreamSynchronize(c->stream));   // the host staging vectors go out of scope
 }
 
-// Runs the network on b (the set-up kernel of the solve has left ||b||^2 partials in partial set
-// 0 when from_setup; otherwise they are computed here) and returns the device guess buffer.
+// Runs the network on b (the solve's set-up kernel has left the ||b||^2 partials in partial set
+// 0), scales its output to the A-norm-optimal multiple, and returns the device guess buffer.
 const float* unet_infer(ds_context* c, const float* b) {
     UNet* u = c->unet;
     require(u != nullptr && u->in4 != nullptr, "no warm-start network loaded (ds_warm_start_load)");
@@ -353,6 +404,8 @@ const float* unet_infer(ds_context* c, const float* b) {
     conv3_kernel<16, 0, 16, MT, true><<<conv_waves(d1), 256, 0, s>>>(d1, u->f2, nullptr, u->wpk[11], u->bias[11], u->f1);
     head_kernel<<<blocks_for(n1), kBlock, 0, s>>>((uint32_t)n1, u->f1, u->head_w, u->bias[12], c->code, u->scale, u->output_scale,
                                                   u->guess);
+    guess_scale_partials_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g, pcg_periodic_bits(c), c->code, u->guess, b, u->scale_partials);
+    guess_scale_apply_kernel<<<kPcgBlocks, kBlock, 0, s>>>((uint32_t)n1, u->scale_partials, kPcgBlocks, u->guess);
     check_launch("warm-start network");
     return u->guess;
 }

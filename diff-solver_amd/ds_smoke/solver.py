@@ -25,7 +25,7 @@ class SolverError(RuntimeError):
 
 
 def load_library(path: str | None = None):
-    """Load the in-tree HIP library (built by __graft_entry__.build())."""
+    """Load the in-tree HIP library (built by __graft_entry__.build()); DS_SMOKE_LIBRARY overrides the path."""
     global _LIB
     if _LIB is not None:
         return _LIB
@@ -33,7 +33,7 @@ def load_library(path: str | None = None):
     # process; loading the library first can leave torch and the library on different runtimes)
     import torch  # noqa: F401
 
-    path = path or abi.library_path()
+    path = path or os.environ.get("DS_SMOKE_LIBRARY") or abi.library_path()
     if not os.path.exists(path):
         raise RuntimeError(f"HIP smoke library not found at {path}; run __graft_entry__.build() first")
     L = C.CDLL(path)
