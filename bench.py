@@ -3,10 +3,12 @@
 
 Workload (one bench step): the 128^3 smoke plume, 64 forward steps, then the VJP of
 0.5 ||rho_64 - target||^2 with respect to the initial state by a reverse sweep with a checkpoint
-every 8 steps (recompute in between). value = simulated fwd+VJP steps per second over the timed
-bench steps, inputs resident in HBM. Also reported: PCG iterations per solve (forward, recompute,
-adjoint), the HBM roofline of the dominant kernel (HIP-event timing of sampled launches inside
-the timed region), and the FP64 CPU oracle timed on the host cores on a bounded sample.
+every 8 steps (recompute in between). Each warm-start mode of the pressure solves (x0 = 0, the
+previous solve's solution, the warm-start network) runs in turn; value = simulated fwd+VJP steps
+per second of the fastest, over the timed bench steps, inputs resident in HBM. Also reported: PCG
+iterations per solve (forward, recompute, adjoint) for every mode, the HBM roofline of the
+dominant kernel (HIP-event timing of sampled launches inside the timed region), and the FP64 CPU
+oracle timed on the host cores on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -104,8 +106,11 @@ def main():
     ap.add_argument("--profile-every", type=int, default=8)
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--warm-start", choices=["none", "previous"], default="none",
-                    help="initial guess of each pressure solve: zero, or the previous solve's solution")
+    ap.add_argument("--warm-start", choices=["all", "none", "previous", "cnn"], default="all",
+                    help="initial guess of each pressure solve: zero, the previous solve's solution, or the warm-start "
+                         "network; all = each in turn (the network when its weights exist), value = the fastest")
+    ap.add_argument("--weights", default=os.path.join(ROOT, "diff-solver_amd", "data", "warm_start_plume.npz"),
+                    help="warm-start network parameters (for --warm-start cnn)")
     ap.add_argument("--cg-variant", choices=["single", "two"], default="single",
                     help="PCG recurrence: single-reduction (one fused kernel per iteration) or two-reduction")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -123,49 +128,67 @@ def main():
     from ds_smoke.solver import Context
 
     sc = scenes.plume(args.grid)
-    warm = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS}[args.warm_start]
     variant = {"single": abi.CG_SINGLE_REDUCTION, "two": abi.CG_TWO_REDUCTION}[args.cg_variant]
-    ctx = Context(sc.config(tolerance=1e-6, warm_start=warm, cg_variant=variant), abi.MODE_DIFFERENTIABLE)
-    s0 = ctx.state_from_host(scenes.initial_state(sc))
-    ctl = ctx.control_from_host(scenes.control(sc))
-    params = ctx.allocate_parameters(sc.params)
-    target = ctx.to_device(sc.target)
-    grad = ctx.allocate_state_adjoint()
+    if args.warm_start == "all":
+        modes = ["none", "previous"] + (["cnn"] if os.path.exists(args.weights) else [])
+    else:
+        modes = [args.warm_start]
 
-    def one():
-        with torch.cuda.stream(ctx.stream):
-            for t in (grad["density"], grad["temperature"]) + tuple(grad["velocity"]):
-                t.zero_()
-        return ctx.loss_gradient(s0, ctl, params, args.sim_steps, args.checkpoint, target, initial_adj=grad)
+    def run_mode(mode):
+        """Warm-up, then the timed bench steps of one warm-start mode in a fresh context."""
+        warm = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS, "cnn": abi.WARM_START_CNN}[mode]
+        ctx = Context(sc.config(tolerance=1e-6, warm_start=warm, cg_variant=variant), abi.MODE_DIFFERENTIABLE)
+        if mode == "cnn":
+            from ds_smoke import warm_start
 
-    for _ in range(args.warmup):
-        one()
-    torch.cuda.synchronize()
-    ctx.profile_enable(args.profile_every)
-    infos = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        loss, _, info = one()
-        infos.append(info)
-    torch.cuda.synchronize()
-    elapsed = time.perf_counter() - t0
-    prof = ctx.profile_read()
-    ctx.profile_enable(0)
+            flat, output_scale, _ = warm_start.load(args.weights)
+            ctx.warm_start_load(flat, output_scale)
+        s0 = ctx.state_from_host(scenes.initial_state(sc))
+        ctl = ctx.control_from_host(scenes.control(sc))
+        params = ctx.allocate_parameters(sc.params)
+        target = ctx.to_device(sc.target)
+        grad = ctx.allocate_state_adjoint()
 
-    sim_steps = args.steps * args.sim_steps
-    info = infos[-1]
-    per_solve = {
-        "forward": info.forward_iterations / max(1, info.forward_solves),
-        "recompute": info.recompute_iterations / max(1, info.recompute_solves),
-        "adjoint": info.adjoint_iterations / max(1, info.adjoint_solves),
-    }
-    n, nf = ctx.domain.cell_count, ctx.domain.unknown_count
-    kern = {}
-    for name, (launches, samples, avg_ms) in prof.items():
-        b = BYTES_UNKNOWN[name] * nf + BYTES_OTHER[name] * (n - nf)
-        kern[name] = {"launches": launches, "samples": samples, "avg_us": avg_ms * 1e3, "algorithmic_bytes": b,
-                      "gbs": b / (avg_ms * 1e-3) / 1e9 if samples else None}
+        def one():
+            with torch.cuda.stream(ctx.stream):
+                for t in (grad["density"], grad["temperature"]) + tuple(grad["velocity"]):
+                    t.zero_()
+            return ctx.loss_gradient(s0, ctl, params, args.sim_steps, args.checkpoint, target, initial_adj=grad)
+
+        for _ in range(args.warmup):
+            one()
+        torch.cuda.synchronize()
+        ctx.profile_enable(args.profile_every)
+        infos = []
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            loss, _, info = one()
+            infos.append(info)
+        torch.cuda.synchronize()
+        elapsed = time.perf_counter() - t0
+        prof = ctx.profile_read()
+        ctx.profile_enable(0)
+        info = infos[-1]
+        per_solve = {
+            "forward": info.forward_iterations / max(1, info.forward_solves),
+            "recompute": info.recompute_iterations / max(1, info.recompute_solves),
+            "adjoint": info.adjoint_iterations / max(1, info.adjoint_solves),
+        }
+        n, nf = ctx.domain.cell_count, ctx.domain.unknown_count
+        kern = {}
+        for name, (launches, samples, avg_ms) in prof.items():
+            b = BYTES_UNKNOWN[name] * nf + BYTES_OTHER[name] * (n - nf)
+            kern[name] = {"launches": launches, "samples": samples, "avg_us": avg_ms * 1e3, "algorithmic_bytes": b,
+                          "gbs": b / (avg_ms * 1e-3) / 1e9 if samples else None}
+        ctx.close()
+        return {"value": args.steps * args.sim_steps / elapsed, "ms_per_step": elapsed / args.steps * 1e3, "loss": loss,
+                "per_solve": per_solve, "kernels": kern, "not_converged": info.not_converged}
+
+    results = {m: run_mode(m) for m in modes}
+    best = max(results, key=lambda m: results[m]["value"])
+    r = results[best]
+    kern = r["kernels"]
     dom = max(kern, key=lambda k: kern[k]["launches"] * (kern[k]["avg_us"] or 0.0)) if kern else None
     roofline = None
     if dom:
@@ -173,14 +196,15 @@ def main():
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
                     "algorithmic_bytes_per_launch": d["algorithmic_bytes"], "avg_launch_us": round(d["avg_us"], 2)}
+    label = {"none": "plain", "previous": "previous", "cnn": "cnn"}
     result = {
         "metric": METRIC,
-        "value": round(sim_steps / elapsed, 4),
+        "value": round(r["value"], 4),
         "unit": "fwd+VJP steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "ms_per_step": round(r["ms_per_step"], 3),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -188,11 +212,11 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"plume{args.grid}_fwd{args.sim_steps}_vjp_checkpoint{args.checkpoint}",
                    "grid": args.grid, "sim_steps_per_bench_step": args.sim_steps,
-                   "checkpoint_interval": args.checkpoint, "cg_tolerance": 1e-6, "warm_start": args.warm_start,
-                   "cg_variant": args.cg_variant,
-                   "pressure_tape": True},
-        "cg_iterations_per_solve": {("plain" if args.warm_start == "none" else args.warm_start): {k: round(v, 1) for k, v in per_solve.items()}},
-        "loss": loss,
+                   "checkpoint_interval": args.checkpoint, "cg_tolerance": 1e-6, "warm_start": label[best],
+                   "cg_variant": args.cg_variant, "pressure_tape": True},
+        "cg_iterations_per_solve": {label[m]: {k: round(v, 1) for k, v in results[m]["per_solve"].items()} for m in modes},
+        "steps_per_s_by_warm_start": {label[m]: round(results[m]["value"], 4) for m in modes},
+        "loss": r["loss"],
         "roofline": roofline,
         "kernels": {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in kern.items()},
     }

@@ -552,25 +552,29 @@ ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], c
         in.extra[a] = extra ? extra[a] : nullptr;
     }
     max_it = std::min(max_it, c->max_iterations);   // the per-iteration history is sized for this cap
+    bool network_guess = false;
     if (x0h == nullptr && c->config.warm_start == DS_WARM_START_CNN) {
         // the network's guess: b first (set-up without a guess), then the set-up again from the guess
         launch_setup(c, kind, in, nullptr, nullptr, false, xh, xl);
         x0h = unet_infer(c, c->pcg.b);
         x0l = nullptr;
         kind = kRhsGiven;
+        network_guess = true;
     }
     const bool warm = x0h != nullptr;
     bool rejected = false;
     launch_setup(c, kind, in, x0h, x0l, x0h == xh, xh, xl);
     double rr0 = 0.0, bb = 0.0;
     if (warm) {
-        // reject a non-finite guess or one whose residual exceeds ||b|| (worse than x0 = 0)
+        // Reject a non-finite guess, and a given guess whose residual exceeds ||b|| (worse than
+        // x0 = 0). The network's guess is scaled to its A-norm-optimal multiple, which never has
+        // more energy error than x0 = 0 (the norm CG minimises), though its residual can be larger.
         pcg_finalize_kernel<<<1, kBlock, 0, c->stream>>>(c->pcg.partials, c->pcg.blocks, c->pcg.status, 1u);
         check_launch("pcg_finalize_kernel");
         const PcgStatus s = read_status(c);
         rr0 = s.rr;
         bb = s.bb;
-        if (!(s.rr <= s.bb)) {
+        if (!std::isfinite(s.rr) || (!network_guess && !(s.rr <= s.bb))) {
             rejected = true;
             launch_setup(c, kRhsGiven, in, nullptr, nullptr, false, xh, xl);
         }

@@ -112,7 +112,8 @@ def generate_pairs(n: int, runs: int, steps: int, seed: int = 100, jitter_cells:
         cache = ctx.allocate_step_cache(pressure=True)
         nxt = ctx.allocate_state()
         if unknown is None:
-            unknown = torch.from_numpy(ctx.masks()[1] > 0).to(ctx.device)
+            code = ctx.masks()[1]
+            unknown = torch.from_numpy(code > 0).to(ctx.device)
         its = 0
         for _ in range(steps):
             ctx.forward_step(state, ctl, params, next_state=nxt, cache=cache)
@@ -125,15 +126,53 @@ def generate_pairs(n: int, runs: int, steps: int, seed: int = 100, jitter_cells:
             state, nxt = nxt, state
         ctx.close()
         log(f"run {r + 1}/{runs}: {len(bs)} pairs")
-    return torch.stack(bs), torch.stack(ps), unknown
+    return torch.stack(bs), torch.stack(ps), unknown, torch.from_numpy(code.astype(np.float32))
 
 
-def train(bs, ps, unknown, epochs: int = 20, batch: int = 2, lr: float = 1e-3, seed: int = 0, time_budget: float = 600.0, log=print):
-    """Adam on the relative L2 error of the guess in the A-norm proxy ||x0 - p|| / ||p||."""
+def apply_matrix(x, diag, per=(False, False, False)):
+    """A x for the pressure matrix on a (..., nz, ny, nx) tensor that is 0 off the unknowns
+    (diag = the per-cell diagonal count, 0 off the unknowns)."""
+    import torch
+
+    y = diag * x
+    for dim, axis in ((-1, 0), (-2, 1), (-3, 2)):
+        if per[axis]:
+            y = y - torch.roll(x, 1, dim) - torch.roll(x, -1, dim)
+        else:
+            n = x.shape[dim]
+            y = y - torch.nn.functional.pad(x.narrow(dim, 0, n - 1), _pad(dim, 1, 0))
+            y = y - torch.nn.functional.pad(x.narrow(dim, 1, n - 1), _pad(dim, 0, 1))
+    return torch.where(diag > 0, y, torch.zeros_like(y))
+
+
+def _pad(dim, before, after):
+    # F.pad takes (last-dim-before, last-dim-after, second-to-last-before, ...)
+    k = -dim - 1
+    return (0, 0) * k + (before, after)
+
+
+def energy_loss(g, b, p, diag):
+    """1 - cos_A(g, p)^2 = 1 - <g, b>^2 / (<g, A g> <b, p>): the share of the solution's energy
+    that the best multiple of g leaves (scale-invariant; 0 for g parallel to p). It weighs the
+    error modes as CG's convergence does."""
+    ag = apply_matrix(g, diag)
+    gb = (g * b).flatten(1).sum(1)
+    gag = (g * ag).flatten(1).sum(1).clamp(min=1e-30)
+    bp = (b * p).flatten(1).sum(1).clamp(min=1e-30)
+    return 1.0 - gb * gb / (gag * bp)
+
+
+def train(bs, ps, unknown, epochs: int = 20, batch: int = 2, lr: float = 1e-3, seed: int = 0, time_budget: float = 600.0, log=print,
+          diag=None, amp: bool = True, init=None):
+    """Adam on the energy loss of the network's guess (see energy_loss); init: flat parameters to start from."""
     import torch
 
     torch.manual_seed(seed)
-    model = build_unet().to(bs.device)
+    model = build_unet()
+    if init is not None:
+        load_parameters(model, init)
+    model = model.to(bs.device).to(memory_format=torch.channels_last_3d)
+    diag = diag.to(bs.device).float()[None]
     # output scale: p / s is O(output_scale)
     with torch.no_grad():
         ratios = []
@@ -151,22 +190,30 @@ def train(bs, ps, unknown, epochs: int = 20, batch: int = 2, lr: float = 1e-3, s
         for i in range(0, n, batch):
             idx = perm[i:i + batch]
             xs, ss = zip(*[network_input(bs[j], unknown) for j in idx.tolist()])
-            x = torch.cat(xs)
-            target = torch.stack([ps[j] / (float(s) * output_scale) for j, s in zip(idx.tolist(), ss)])[:, None]
-            mask = unknown[None, None].float()
-            out = model(x) * mask
-            err = ((out - target * mask) ** 2).flatten(1).sum(1) / (target ** 2).flatten(1).sum(1).clamp(min=1e-30)
-            loss = err.mean()
+            x = torch.cat(xs).contiguous(memory_format=torch.channels_last_3d)
+            try:
+                with torch.autocast("cuda", dtype=torch.bfloat16, enabled=amp):
+                    out = model(x)
+            except RuntimeError as e:   # no BF16 kernels for these convolutions: train in FP32
+                if not amp:
+                    raise
+                log(f"BF16 autocast failed ({e}); training in FP32")
+                amp = False
+                out = model(x)
+            g = out.float()[:, 0] * unknown[None].float()
+            bn = torch.stack([bs[j] / float(s) for j, s in zip(idx.tolist(), ss)])
+            pn = torch.stack([ps[j] / float(s) for j, s in zip(idx.tolist(), ss)])
+            loss = energy_loss(g, bn, pn, diag).mean()
             opt.zero_grad()
             loss.backward()
             opt.step()
-            total += float(loss) * len(idx)
+            total += loss.item() * len(idx)
             step += 1
             if step % 20 == 0:
-                log(f"step {step}: batch relative squared error {float(loss):.4f} ({time.time() - t0:.0f} s)")
+                log(f"step {step}: batch energy loss {loss.item():.4f} ({time.time() - t0:.0f} s)")
             if time.time() - t0 > time_budget:
                 break
-        log(f"epoch {ep + 1}: relative squared error {total / max(1, min(n, i + batch)):.4f} ({time.time() - t0:.0f} s)")
+        log(f"epoch {ep + 1}: energy loss {total / max(1, min(n, i + batch)):.4f} ({time.time() - t0:.0f} s)")
         if time.time() - t0 > time_budget:
             break
     return model, output_scale

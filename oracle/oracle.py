@@ -103,7 +103,10 @@ class Oracle:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().oracle_destroy(C.c_void_p(self.h))
+            try:
+                lib().oracle_destroy(C.c_void_p(self.h))
+            except TypeError:   # interpreter shutdown: module globals are already gone
+                pass
             self.h = None
 
     # -- shapes

@@ -41,3 +41,27 @@ def test_flat_round_trip_and_input_scale():
     v = inp[0, 0][unknown]
     assert abs(float((v.double() ** 2).mean()) - 1.0) < 1e-5
     assert torch.equal(inp[0, 1], unknown.float())
+
+
+def test_training_matrix_and_energy_loss_against_oracle():
+    """The training loss's A (torch) equals the oracle's pressure matrix on a closed box; the energy
+    loss is 0 for the exact solution, scale-invariant, and in (0, 1] otherwise."""
+    import oracle as O
+    import torch
+    from ds_smoke import scenes, warm_start
+
+    sc = scenes.plume(8, with_target=False)
+    orc = O.Oracle(sc, tolerance=1e-13)
+    _, code, _ = orc.masks()
+    rng = np.random.default_rng(4)
+    x = np.where(code > 0, rng.standard_normal(code.shape), 0.0)
+    diag = torch.from_numpy(code.astype(np.float64))
+    ax = warm_start.apply_matrix(torch.from_numpy(x), diag).numpy()
+    np.testing.assert_allclose(ax, orc.apply_matrix(x), rtol=1e-12, atol=1e-12)
+    b = np.where(code > 0, rng.standard_normal(code.shape), 0.0)
+    p, _ = orc.pcg(b, tol=1e-13)
+    bt, pt = torch.from_numpy(b)[None], torch.from_numpy(p)[None]
+    assert float(warm_start.energy_loss(pt, bt, pt, diag)) < 1e-9
+    assert float(warm_start.energy_loss(-3.0 * pt, bt, pt, diag)) < 1e-9
+    r = float(warm_start.energy_loss(torch.from_numpy(x)[None], bt, pt, diag))
+    assert 0.0 < r <= 1.0
