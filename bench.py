@@ -27,10 +27,10 @@ HBM_PEAK_GBS = 8000.0
 # their FP32 vectors and the u8 code once; other cells only read the code (and the direction
 # kernel writes p = 0 there).
 # pcg_fused_kernel (single-reduction form, one launch per iteration): reads code, r, w, s, p and
-# the solution pair, writes r, w, s, p and the pair on unknowns; other cells read the code and
-# write zeros into the next r and w.
+# the solution increment, writes r, w, s, p and the increment on unknowns; other cells read the
+# code and write zeros into the next r and w.
 BYTES_UNKNOWN = {"pcg_direction_kernel": 4 + 4 + 1 + 4, "pcg_update_kernel": 4 + 4 + 1 + 8 + 4 + 8,
-                 "pcg_fused_kernel": 1 + 4 * 4 + 8 + 4 * 4 + 8}
+                 "pcg_fused_kernel": 1 + 4 * 4 + 4 + 4 * 4 + 4}
 BYTES_OTHER = {"pcg_direction_kernel": 1 + 4, "pcg_update_kernel": 1, "pcg_fused_kernel": 1 + 4 + 4}
 
 

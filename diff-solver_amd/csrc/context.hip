@@ -390,6 +390,8 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
             // s is written on unknowns only; its other cells must read as 0 (as halo values)
             c->pcg.r_alt = device_alloc<float>(c, n + 1);
             fill_zero(c, c->pcg.r_alt, n + 1);
+            c->pcg.xd = device_alloc<float>(c, n + 1);
+            fill_zero(c, c->pcg.xd, n + 1);
             for (int k = 0; k < 2; ++k) {
                 c->pcg.wv[k] = device_alloc<float>(c, n + 1);
                 c->pcg.sv[k] = device_alloc<float>(c, n + 1);

@@ -134,6 +134,7 @@ struct PcgWork {
     float* r_alt = nullptr;
     float* wv[2]{};
     float* sv[2]{};
+    float* xd = nullptr;         // solution increment since the last fold into the pair
     double* partials = nullptr;  // two sets of kPartCount * kPcgMaxBlocks
     double* hist = nullptr;      // per-iteration scalars: rz[k], pq[k] (2 * (max_it + 2))
     PcgStatus* status = nullptr; // device

@@ -243,6 +243,22 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fused_replace_kernel(Geo g, u
     if (threadIdx.x == 0) out_parts[kPartRR * kPcgMaxBlocks + lb] = v[0];
 }
 
+// x_hi + x_lo += x_d (compensated), x_d = 0: folds the single-reduction iterations' increment
+// into the solution pair.
+__global__ __launch_bounds__(kPcgThreads) void pcg_fold_kernel(uint32_t n, float* __restrict__ xh, float* __restrict__ xl,
+                                                               float* __restrict__ xd, PcgStatus* status, uint32_t respect_stop) {
+    if (respect_stop && status->stop) return;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float d = xd[i];
+        if (d == 0.0f) continue;
+        float h = xh[i], l = xl[i];
+        compensated_add(h, l, d);
+        xh[i] = h;
+        xl[i] = l;
+        xd[i] = 0.0f;
+    }
+}
+
 // One block: reduce the ||b||^2 (optional) and ||r||^2 partials into the status.
 __global__ __launch_bounds__(256) void pcg_finalize_kernel(const double* __restrict__ partials, uint32_t G,
                                                            PcgStatus* status, uint32_t with_bb) {
@@ -348,8 +364,7 @@ ds_solve_info pcg_iterate_fused(ds_context* c, float* xh, float* xl, double tol,
         v.w_out = prime ? w.wv[set] : w.wv[set ^ 1];
         v.s_out = w.sv[set ^ 1];
         v.p = w.p[0];
-        v.xh = xh;
-        v.xl = xl;
+        v.xd = w.xd;
         return v;
     };
     // with_ps: a previous direction exists (k > 0), so the denominator products are formed too
@@ -358,7 +373,11 @@ ds_solve_info pcg_iterate_fused(ds_context* c, float* xh, float* xl, double tol,
         v.s = w.sv[set];   // s_{i-1}: the replaced s of this set
         launch_fused<kFusedPrime>(c, per, v, nullptr, nullptr, parts[set], tol2, 0u, 0u, respect_stop, with_ps ? 1u : 0u);
     };
+    auto fold = [&](uint32_t respect_stop) {
+        pcg_fold_kernel<<<w.blocks, kPcgThreads, 0, c->stream>>>(c->geo.n, xh, xl, w.xd, w.status, respect_stop);
+    };
     auto replace = [&](int set, bool with_s, uint32_t respect_stop) {
+        fold(respect_stop);
         pcg_fused_replace_kernel<<<w.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, c->code, w.b, xh, xl, w.p[0], rv[set], w.sv[set],
                                                                           with_s ? 1u : 0u, parts[set], w.status, respect_stop);
     };
@@ -366,6 +385,7 @@ ds_solve_info pcg_iterate_fused(ds_context* c, float* xh, float* xl, double tol,
     info.warm_started = warm ? 1u : 0u;
     info.guess_rejected = rejected ? 1u : 0u;
     int set = 0;
+    fill_zero(c, w.xd, c->geo.n);
     prime(0, 0u, false);
     check_launch("pcg_fused_kernel (prime)");
     uint32_t k = 0;

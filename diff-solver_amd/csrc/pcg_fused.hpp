@@ -24,6 +24,12 @@
 // double-buffered (set c is read, set c^1 written); p and x are read and written by their own
 // cell only and stay single. The per-block FP64 partials are double-buffered the same way.
 //
+// The solution is the compensated pair x_hi + x_lo plus an FP32 increment x_d that the iterations
+// update (x_d += alpha p); pcg_fold_kernel adds x_d into the pair and clears it at every residual
+// replacement and at exit. x_d holds at most one replacement interval of steps, so its rounding is
+// relative to those steps, not to x, and the pair keeps the accuracy it needs (FP32 x alone stalls
+// above 1e-6); each iteration touches 4 bytes of solution instead of the pair's 8.
+//
 // Layout: the plane march of the two-kernel path — a 1024-thread block owns a 64 x 16 (x, y)
 // tile and a chunk of z planes; u of the plane being differentiated sits in LDS (own cell plus
 // the tile rim), its z neighbours in registers. The raw loads of plane z+1 are consumed entirely
@@ -48,8 +54,7 @@ struct FusedVecs {
     float* w_out;         // w_{i+1}      (set c^1; the priming pass writes set c)
     float* s_out;         // s_i          (set c^1)
     float* p;             // p_{i-1} -> p_i (in place)
-    float* xh;            // solution pair (in place)
-    float* xl;
+    float* xd;            // FP32 increment of the solution since the last fold into the pair (in place)
 };
 
 // Halo values of one cell: enough to form u_{i+1} there.
@@ -57,9 +62,10 @@ struct FHalo {
     float r, w, s;
     uint8_t d;   // kept as loaded: a widening right after the load would wait for it
 };
-// Own-cell values only the owning thread needs (s: the priming pass's s_{i-1}).
+// Own-cell values only the owning thread needs (d: the solution increment; s: the priming
+// pass's s_{i-1}).
 struct FOwn {
-    float p, h, l, s;
+    float p, d, s;
 };
 struct FPlane {
     FHalo c, x1, y2;
@@ -149,14 +155,13 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fused_kernel(Geo g, uint32_t 
         P.c = fused_load_halo<Mode>(halo_index(g, j), v, code);
         const uint32_t jo = j >= 0 ? (uint32_t)j : 0u;
         if (kUpdate) {
-            P.o.h = v.xh[jo];
-            P.o.l = v.xl[jo];
+            P.o.d = v.xd[jo];
             P.o.p = Mode == kFusedNext ? v.p[jo] : 0.0f;
             P.o.s = 0.0f;
         } else if (with_ps) {   // block-uniform
-            P.o = FOwn{v.p[jo], 0.0f, 0.0f, v.s[jo]};
+            P.o = FOwn{v.p[jo], 0.0f, v.s[jo]};
         } else {
-            P.o = FOwn{0.0f, 0.0f, 0.0f, 0.0f};
+            P.o = FOwn{0.0f, 0.0f, 0.0f};
         }
     };
 
@@ -217,10 +222,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fused_kernel(Geo g, uint32_t 
                     const float ui = P.c.r * inv_diag(P.c.d);
                     const float pi = Mode == kFusedNext ? __fmaf_rn(beta, P.o.p, ui) : ui;
                     v.p[i] = pi;
-                    float h = P.o.h, l = P.o.l;
-                    compensated_add(h, l, alpha * pi);
-                    v.xh[i] = h;
-                    v.xl[i] = l;
+                    v.xd[i] = __fmaf_rn(alpha, pi, P.o.d);
                     v.s_out[i] = si;
                     pz = pi;
                     su += (double)si * u;
