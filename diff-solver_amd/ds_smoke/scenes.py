@@ -88,6 +88,7 @@ class Scene:
     temperature_modes: tuple = (1, 1, 1, 1, 1, 1)
     temperature_values: tuple = (0.0,) * 6
     name: str = "plume"
+    vorticity: bool = False                 # vorticity_confinement_enabled; strength = params[3]
 
     @property
     def cell_count(self) -> int:
@@ -119,7 +120,7 @@ class Scene:
         cfg._collider_keepalive = arr  # keep the array alive with the struct
         cfg.colliders = C.cast(arr, C.POINTER(abi.Collider))
         cfg.collider_count = len(self.colliders)
-        cfg.vorticity_confinement_enabled = 0
+        cfg.vorticity_confinement_enabled = 1 if self.vorticity else 0
         cfg.projection_solver = solver
         cfg.cg_max_iterations = max_iterations
         cfg.cg_tolerance = tolerance

@@ -51,6 +51,10 @@ def lib():
         L.oracle_set_solver.argtypes = [C.c_void_p, C.c_uint32, C.c_double, C.c_uint32, C.c_uint32]
         L.oracle_pcg.argtypes = [C.c_void_p, DP, DP, DP, C.c_double, C.c_uint32, C.POINTER(abi.SolveInfo)]
         L.oracle_rbgs.argtypes = [C.c_void_p, DP, DP, C.c_uint32]
+        L.oracle_vorticity_forward.restype = None
+        L.oracle_vorticity_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
+        L.oracle_vorticity_vjp.restype = None
+        L.oracle_vorticity_vjp.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, DP]
         L.oracle_set_threads.restype = C.c_int
         L.oracle_set_threads.argtypes = [C.c_int]
         _lib = L
@@ -164,6 +168,17 @@ class Oracle:
         lib().oracle_buoyancy_vjp(C.c_void_p(self.h), _p(f64(rho)), _p(f64(T)), _p(f64(params)),
                                   _arr3([f64(x) for x in fa]), _p(rho_adj), _p(T_adj), _arr3(e_adj), _p(param_adj))
 
+    def vorticity_forward(self, u, confinement, f=None):
+        """f + confinement * h * (N x w) on fluid cells (f defaults to zero force)."""
+        f = tuple(self.zeros_cell() for _ in range(3)) if f is None else tuple(f64(x).copy() for x in f)
+        lib().oracle_vorticity_forward(C.c_void_p(self.h), _arr3([f64(x) for x in u]), float(confinement), _arr3(f))
+        return f
+
+    def vorticity_vjp(self, u, confinement, fa, ua, conf_adj):
+        """Accumulates the velocity adjoint into ua (faces) and the confinement adjoint into conf_adj[0]."""
+        lib().oracle_vorticity_vjp(C.c_void_p(self.h), _arr3([f64(x) for x in u]), float(confinement),
+                                   _arr3([f64(x) for x in fa]), _arr3(ua), _p(conf_adj))
+
     def integrate_forward(self, u, f):
         out = self.zeros_faces()
         lib().oracle_integrate_forward(C.c_void_p(self.h), _arr3([f64(x) for x in u]), _arr3([f64(x) for x in f]), _arr3(out))
@@ -259,7 +274,11 @@ class Oracle:
               "forced_velocity": tuple(cache[5:8]), "advected_velocity": tuple(cache[8:11]), "pressure": cache[11]}
         return ns, ch, info
 
-    def vjp_step(self, next_state, cache, params, next_adj, prev_adj=None, control_adj=None, param_adj=None):
+    def vjp_step(self, next_state, cache, params, next_adj, prev_adj=None, control_adj=None, param_adj=None, state=None):
+        """state: the step's input (its velocity is read with vorticity confinement on)."""
+        st_arr = None
+        if state is not None:
+            st_arr = [f64(state["density"]), f64(state["temperature"])] + [f64(v) for v in state["velocity"]]
         nxt = [f64(next_state["density"]), f64(next_state["temperature"])] + [f64(v) for v in next_state["velocity"]]
         ch = [f64(cache["sourced_density"]), f64(cache["sourced_temperature"]), None, None, None] + \
             [f64(v) for v in cache["forced_velocity"]] + [None, None, None, None]
@@ -273,9 +292,9 @@ class Oracle:
         if param_adj is None:
             param_adj = np.zeros(4)
         info = abi.SolveInfo()
-        st = lib().oracle_vjp_step(C.c_void_p(self.h), _arrn(nxt), _arrn(ch), _p(f64(params)), _arrn(na), _arrn(pa),
+        st = lib().oracle_vjp_step(C.c_void_p(self.h), _arrn(st_arr) if st_arr is not None else None, _arrn(nxt), _arrn(ch), _p(f64(params)), _arrn(na), _arrn(pa),
                                    _arrn(ca) if ca is not None else None, _p(param_adj), C.byref(info))
-        assert st == 0
+        assert st == 0, lib().oracle_last_error()
         return prev_adj, info
 
     def loss_gradient(self, state, control, params, steps, k, target, want_control=False):

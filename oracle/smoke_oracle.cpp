@@ -74,6 +74,7 @@ struct Oracle {
     i64 anchor = 0;
     std::uint32_t extra_anchors = 0;
     bool fixed_pressure = false;
+    bool vorticity = false;            // vorticity_confinement_enabled (data.ixx:82)
     i64 unknowns = 0, solids = 0;
     std::uint32_t projection_solver = DS_PROJECTION_PCG;
     std::uint32_t rbgs_iterations = 80;
@@ -342,6 +343,181 @@ void buoyancy_vjp(const Oracle& o, const double* rho, const double* T, const dou
         param_adj[1] += ar;
         param_adj[2] += at;
     }
+}
+
+// ---- vorticity confinement (operators.cu:662-846, launchers :973-1002) ----------------------
+// smooth_epsilon, operators.cu:12 (an FP32 constant there; its FP32 value is used here).
+constexpr double kSmoothEpsilon = (double)1.0e-6F;
+
+// centered_load / centered_scatter, operators.cu:662-674: neighbours clamp to the grid (never
+// wrap, whatever the boundary modes).
+inline i64 clamped_cell(const Oracle& o, int x, int y, int z) {
+    x = std::max(0, std::min(o.nx - 1, x));
+    y = std::max(0, std::min(o.ny - 1, y));
+    z = std::max(0, std::min(o.nz - 1, z));
+    return cell(o, x, y, z);
+}
+
+// Cached fields of vorticity_forward (the reference's VorticityCache): cell-centred velocity,
+// vorticity, smoothed magnitude sqrt(|w|^2 + eps^2), normal N = grad|w| / len and its
+// normaliser len = sqrt(|grad|w||^2 + eps^2). Solid cells: c = w = N = 0, magnitude = len = eps.
+struct VorticityFields {
+    std::vector<double> c[3], w[3], mag, nrm[3], len;
+};
+
+void vorticity_fields(const Oracle& o, const double* const u[3], VorticityFields& v) {
+    const i64 n = o.n;
+    for (int a = 0; a < 3; ++a) {
+        v.c[a].assign(n, 0.0);
+        v.w[a].assign(n, 0.0);
+        v.nrm[a].assign(n, 0.0);
+    }
+    v.mag.assign(n, kSmoothEpsilon);
+    v.len.assign(n, kSmoothEpsilon);
+    // centered_velocity_kernel, operators.cu:678-687
+    for (int z = 0; z < o.nz; ++z)
+        for (int y = 0; y < o.ny; ++y)
+            for (int x = 0; x < o.nx; ++x) {
+                const i64 i = cell(o, x, y, z);
+                if (o.mask[i] != 0) continue;
+                v.c[0][i] = 0.5 * (u[0][idx3(x, y, z, o.nx + 1, o.ny)] + u[0][idx3(x + 1, y, z, o.nx + 1, o.ny)]);
+                v.c[1][i] = 0.5 * (u[1][idx3(x, y, z, o.nx, o.ny + 1)] + u[1][idx3(x, y + 1, z, o.nx, o.ny + 1)]);
+                v.c[2][i] = 0.5 * (u[2][idx3(x, y, z, o.nx, o.ny)] + u[2][idx3(x, y, z + 1, o.nx, o.ny)]);
+            }
+    const double s = 0.5 / o.h;
+    auto C = [&](int a, int x, int y, int z) { return v.c[a][clamped_cell(o, x, y, z)]; };
+    // curl_magnitude_kernel, operators.cu:689-701
+    for (int z = 0; z < o.nz; ++z)
+        for (int y = 0; y < o.ny; ++y)
+            for (int x = 0; x < o.nx; ++x) {
+                const i64 i = cell(o, x, y, z);
+                if (o.mask[i] != 0) continue;
+                const double wx = (C(2, x, y + 1, z) - C(2, x, y - 1, z) - C(1, x, y, z + 1) + C(1, x, y, z - 1)) * s;
+                const double wy = (C(0, x, y, z + 1) - C(0, x, y, z - 1) - C(2, x + 1, y, z) + C(2, x - 1, y, z)) * s;
+                const double wz = (C(1, x + 1, y, z) - C(1, x - 1, y, z) - C(0, x, y + 1, z) + C(0, x, y - 1, z)) * s;
+                v.w[0][i] = wx;
+                v.w[1][i] = wy;
+                v.w[2][i] = wz;
+                v.mag[i] = std::sqrt(wx * wx + wy * wy + wz * wz + kSmoothEpsilon * kSmoothEpsilon);
+            }
+    // vorticity_normal_kernel, operators.cu:703-720 (solid neighbours contribute their eps)
+    auto M = [&](int x, int y, int z) { return v.mag[clamped_cell(o, x, y, z)]; };
+    for (int z = 0; z < o.nz; ++z)
+        for (int y = 0; y < o.ny; ++y)
+            for (int x = 0; x < o.nx; ++x) {
+                const i64 i = cell(o, x, y, z);
+                if (o.mask[i] != 0) continue;
+                const double gx = (M(x + 1, y, z) - M(x - 1, y, z)) * s;
+                const double gy = (M(x, y + 1, z) - M(x, y - 1, z)) * s;
+                const double gz = (M(x, y, z + 1) - M(x, y, z - 1)) * s;
+                const double len = std::sqrt(gx * gx + gy * gy + gz * gz + kSmoothEpsilon * kSmoothEpsilon);
+                v.nrm[0][i] = gx / len;
+                v.nrm[1][i] = gy / len;
+                v.nrm[2][i] = gz / len;
+                v.len[i] = len;
+            }
+}
+
+// vorticity_forward (operators.cu:973-979): f += confinement * h * (N x w) on fluid cells
+// (add_vorticity_force_kernel, :722-729). u is the step's input velocity.
+void vorticity_forward(const Oracle& o, const double* const u[3], double confinement, double* const f[3]) {
+    VorticityFields v;
+    vorticity_fields(o, u, v);
+    const double scale = confinement * o.h;
+    for (i64 i = 0; i < o.n; ++i) {
+        if (o.mask[i] != 0) continue;
+        const double N0 = v.nrm[0][i], N1 = v.nrm[1][i], N2 = v.nrm[2][i];
+        const double w0 = v.w[0][i], w1 = v.w[1][i], w2 = v.w[2][i];
+        f[0][i] += scale * (N1 * w2 - N2 * w1);
+        f[1][i] += scale * (N2 * w0 - N0 * w2);
+        f[2][i] += scale * (N0 * w1 - N1 * w0);
+    }
+}
+
+// vorticity_vjp (operators.cu:990-1002): force reverse (:781-800), normal reverse (:802-819),
+// magnitude reverse (:821-828), curl reverse (:830-844), centered-velocity reverse (:846-856).
+// Accumulates into the velocity adjoint ua (faces) and conf_adj (one double).
+void vorticity_vjp(const Oracle& o, const double* const u[3], double confinement, const double* const fa[3],
+                   double* const ua[3], double* conf_adj) {
+    VorticityFields v;
+    vorticity_fields(o, u, v);
+    const i64 n = o.n;
+    std::vector<double> wa[3], na[3], ca[3], ma(n, 0.0);
+    for (int a = 0; a < 3; ++a) {
+        wa[a].assign(n, 0.0);
+        na[a].assign(n, 0.0);
+        ca[a].assign(n, 0.0);
+    }
+    const double scale = confinement * o.h;
+    double ctot = 0.0;
+    for (i64 i = 0; i < n; ++i) {
+        if (o.mask[i] != 0) continue;
+        const double N0 = v.nrm[0][i], N1 = v.nrm[1][i], N2 = v.nrm[2][i];
+        const double w0 = v.w[0][i], w1 = v.w[1][i], w2 = v.w[2][i];
+        const double f0 = fa[0][i], f1 = fa[1][i], f2 = fa[2][i];
+        ctot += o.h * ((N1 * w2 - N2 * w1) * f0 + (N2 * w0 - N0 * w2) * f1 + (N0 * w1 - N1 * w0) * f2);
+        na[0][i] += scale * (w1 * f2 - w2 * f1);   // w x f_bar
+        na[1][i] += scale * (w2 * f0 - w0 * f2);
+        na[2][i] += scale * (w0 * f1 - w1 * f0);
+        wa[0][i] += scale * (f1 * N2 - f2 * N1);   // f_bar x N
+        wa[1][i] += scale * (f2 * N0 - f0 * N2);
+        wa[2][i] += scale * (f0 * N1 - f1 * N0);
+    }
+    if (conf_adj) *conf_adj += ctot;
+    const double s = 0.5 / o.h;
+    auto scatter = [&](std::vector<double>& dst, int x, int y, int z, double value) { dst[clamped_cell(o, x, y, z)] += value; };
+    for (int z = 0; z < o.nz; ++z)
+        for (int y = 0; y < o.ny; ++y)
+            for (int x = 0; x < o.nx; ++x) {
+                const i64 i = cell(o, x, y, z);
+                if (o.mask[i] != 0) continue;
+                const double proj = v.nrm[0][i] * na[0][i] + v.nrm[1][i] * na[1][i] + v.nrm[2][i] * na[2][i];
+                const double gx = (na[0][i] - v.nrm[0][i] * proj) / v.len[i];
+                const double gy = (na[1][i] - v.nrm[1][i] * proj) / v.len[i];
+                const double gz = (na[2][i] - v.nrm[2][i] * proj) / v.len[i];
+                scatter(ma, x + 1, y, z, s * gx);
+                scatter(ma, x - 1, y, z, -s * gx);
+                scatter(ma, x, y + 1, z, s * gy);
+                scatter(ma, x, y - 1, z, -s * gy);
+                scatter(ma, x, y, z + 1, s * gz);
+                scatter(ma, x, y, z - 1, -s * gz);
+            }
+    for (i64 i = 0; i < n; ++i) {
+        if (o.mask[i] != 0) continue;
+        const double k = ma[i] / v.mag[i];
+        for (int a = 0; a < 3; ++a) wa[a][i] += k * v.w[a][i];
+    }
+    for (int z = 0; z < o.nz; ++z)
+        for (int y = 0; y < o.ny; ++y)
+            for (int x = 0; x < o.nx; ++x) {
+                const i64 i = cell(o, x, y, z);
+                if (o.mask[i] != 0) continue;
+                const double ax = wa[0][i] * s, ay = wa[1][i] * s, az = wa[2][i] * s;
+                scatter(ca[2], x, y + 1, z, ax);
+                scatter(ca[2], x, y - 1, z, -ax);
+                scatter(ca[1], x, y, z + 1, -ax);
+                scatter(ca[1], x, y, z - 1, ax);
+                scatter(ca[0], x, y, z + 1, ay);
+                scatter(ca[0], x, y, z - 1, -ay);
+                scatter(ca[2], x + 1, y, z, -ay);
+                scatter(ca[2], x - 1, y, z, ay);
+                scatter(ca[1], x + 1, y, z, az);
+                scatter(ca[1], x - 1, y, z, -az);
+                scatter(ca[0], x, y + 1, z, -az);
+                scatter(ca[0], x, y - 1, z, az);
+            }
+    for (int z = 0; z < o.nz; ++z)
+        for (int y = 0; y < o.ny; ++y)
+            for (int x = 0; x < o.nx; ++x) {
+                const i64 i = cell(o, x, y, z);
+                if (o.mask[i] != 0) continue;
+                ua[0][idx3(x, y, z, o.nx + 1, o.ny)] += 0.5 * ca[0][i];
+                ua[0][idx3(x + 1, y, z, o.nx + 1, o.ny)] += 0.5 * ca[0][i];
+                ua[1][idx3(x, y, z, o.nx, o.ny + 1)] += 0.5 * ca[1][i];
+                ua[1][idx3(x, y + 1, z, o.nx, o.ny + 1)] += 0.5 * ca[1][i];
+                ua[2][idx3(x, y, z, o.nx, o.ny)] += 0.5 * ca[2][i];
+                ua[2][idx3(x, y, z + 1, o.nx, o.ny)] += 0.5 * ca[2][i];
+            }
 }
 
 // averaged_center_force + integrate_kernel, operators.cu:339-364
@@ -833,6 +1009,7 @@ int forward_step(Oracle& o, const double* const s[5], const double* const c[5], 
     const double* e[3]{c[2], c[3], c[4]};
     buoyancy_forward(o, rs, ts, e, params, f);
     const double* u[3]{s[2], s[3], s[4]};
+    if (o.vorticity) vorticity_forward(o, u, params[3], f);   // ForceAssemblyOperator, operators.cpp:125-128
     const double* fc[3]{f[0], f[1], f[2]};
     integrate_forward(o, u, fc, us);
     const double* usc[3]{us[0], us[1], us[2]};
@@ -849,10 +1026,15 @@ int forward_step(Oracle& o, const double* const s[5], const double* const c[5], 
     return DS_OK;
 }
 
-// Model::vjp_step, model.cpp:361-389 (reverse order of forward_step).
-int vjp_step(Oracle& o, const double* const next[5], const double* const cache[12], const double* params,
-             const double* const next_adj[5], double* const prev_adj[5], double* const control_adj[5],
+// Model::vjp_step, model.cpp:361-389 (reverse order of forward_step). state (the step's input) is
+// read only for its velocity, and only with vorticity confinement on.
+int vjp_step(Oracle& o, const double* const state[5], const double* const next[5], const double* const cache[12],
+             const double* params, const double* const next_adj[5], double* const prev_adj[5], double* const control_adj[5],
              double* param_adj, ds_solve_info* info) {
+    if (o.vorticity && (!state || !state[2] || !state[3] || !state[4])) {
+        g_error = "vjp_step: vorticity confinement needs the step's input velocity";
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
     const i64 n = o.n;
     std::vector<double> rsa(n, 0.0), tsa(n, 0.0), fa[3], pua[3], aua[3], rua[3], fua[3];
     for (int a = 0; a < 3; ++a) {
@@ -891,6 +1073,10 @@ int vjp_step(Oracle& o, const double* const next[5], const double* const cache[1
             ea[a] = e_sink[a].data();
         }
     buoyancy_vjp(o, cache[0], cache[1], params, fac, rsa.data(), tsa.data(), ea, param_adj);
+    if (o.vorticity) {   // ForceAssemblyOperator::vjp, operators.cpp:135-138
+        const double* u[3]{state[2], state[3], state[4]};
+        vorticity_vjp(o, u, params[3], fac, pv, param_adj ? param_adj + 3 : nullptr);
+    }
     source_vjp(o, rsa.data(), prev_adj[0], control_adj ? control_adj[0] : nullptr);
     source_vjp(o, tsa.data(), prev_adj[1], control_adj ? control_adj[1] : nullptr);
     return DS_OK;
@@ -1006,7 +1192,10 @@ int loss_gradient(Oracle& o, const double* const init[5], const double* const co
             ds_solve_info si{};
             const double* npc[5]{np[0], np[1], np[2], np[3], np[4]};
             const double* rac[5]{ra[0], ra[1], ra[2], ra[3], ra[4]};
-            const int st = vjp_step(o, npc, cp, params, rac, pa, control_adj, param_adj, &si);
+            double* sp[5];
+            ptrs(states[s - s0], sp);
+            const double* spc[5]{sp[0], sp[1], sp[2], sp[3], sp[4]};
+            const int st = vjp_step(o, spc, npc, cp, params, rac, pa, control_adj, param_adj, &si);
             if (st != DS_OK) return st;
             record(ti, si, 2);
             running = std::move(prev);
@@ -1057,6 +1246,7 @@ int build(Oracle& o, const ds_config& cfg) {
     for (int k = 0; k < 18; ++k) o.vb.values[k] = cfg.velocity_boundary.values[k];
     o.projection_solver = cfg.projection_solver;
     o.rbgs_iterations = cfg.pressure_iterations;
+    o.vorticity = cfg.vorticity_confinement_enabled != 0;
     o.tolerance = cfg.cg_tolerance > 0.0 ? cfg.cg_tolerance : 1e-6;
     o.max_iterations = cfg.cg_max_iterations ? cfg.cg_max_iterations : 4u * (cfg.resolution[0] + cfg.resolution[1] + cfg.resolution[2]) + 1000u;
     o.n = (i64)o.nx * o.ny * o.nz;
@@ -1260,6 +1450,13 @@ void oracle_buoyancy_vjp(void* h, const double* rho, const double* T, const doub
                          double* rho_adj, double* T_adj, double* const e_adj[3], double* param_adj) {
     buoyancy_vjp(*static_cast<Oracle*>(h), rho, T, params, fa, rho_adj, T_adj, e_adj, param_adj);
 }
+void oracle_vorticity_forward(void* h, const double* const u[3], double confinement, double* const f[3]) {
+    vorticity_forward(*static_cast<Oracle*>(h), u, confinement, f);
+}
+void oracle_vorticity_vjp(void* h, const double* const u[3], double confinement, const double* const fa[3],
+                          double* const ua[3], double* conf_adj) {
+    vorticity_vjp(*static_cast<Oracle*>(h), u, confinement, fa, ua, conf_adj);
+}
 void oracle_integrate_forward(void* h, const double* const u[3], const double* const f[3], double* const out[3]) {
     integrate_forward(*static_cast<Oracle*>(h), u, f, out);
 }
@@ -1305,10 +1502,10 @@ int oracle_forward_step(void* h, const double* const s[5], const double* const c
                         double* const next[5], double* const cache[12], const double* guess, ds_solve_info* info) {
     return forward_step(*static_cast<Oracle*>(h), s, c, params, next, cache, guess, info);
 }
-int oracle_vjp_step(void* h, const double* const next[5], const double* const cache[12], const double* params,
-                    const double* const next_adj[5], double* const prev_adj[5], double* const control_adj[5],
-                    double* param_adj, ds_solve_info* info) {
-    return vjp_step(*static_cast<Oracle*>(h), next, cache, params, next_adj, prev_adj, control_adj, param_adj, info);
+int oracle_vjp_step(void* h, const double* const state[5], const double* const next[5], const double* const cache[12],
+                    const double* params, const double* const next_adj[5], double* const prev_adj[5],
+                    double* const control_adj[5], double* param_adj, ds_solve_info* info) {
+    return vjp_step(*static_cast<Oracle*>(h), state, next, cache, params, next_adj, prev_adj, control_adj, param_adj, info);
 }
 int oracle_loss_gradient(void* h, const double* const init[5], const double* const control[5], const double* params,
                          std::uint32_t steps, std::uint32_t k, const double* target, double* const final_state[5],
