@@ -6,7 +6,8 @@
 // Header-only over the C ABI in ds/ds_smoke.h. Differences from the reference, by design:
 //   * adjoint fields are FP32 (adjoint_t), parameter adjoints FP64;
 //   * Configuration adds the pressure solver fields (projection_solver, cg_tolerance,
-//     cg_max_iterations, warm_start) and defaults vorticity confinement to off (not built);
+//     cg_max_iterations, warm_start, cg_variant); vorticity confinement defaults to on, as in
+//     the reference (data.ixx:82);
 //   * errors surface as std::runtime_error carrying ds_last_error() (as check_launch did);
 //   * StepCache carries an optional pressure pair used to warm-start a recompute.
 #pragma once
@@ -138,7 +139,7 @@ struct Configuration {
     ScalarBoundary density_boundary{};
     ScalarBoundary temperature_boundary{};
     std::vector<Collider> colliders{};
-    bool vorticity_confinement_enabled{false};
+    bool vorticity_confinement_enabled{true};   // data.ixx:82
     ProjectionSolver projection_solver{ProjectionSolver::pcg};
     double cg_tolerance{1e-6};
     std::uint32_t cg_max_iterations{0u};

@@ -13,6 +13,7 @@
 
 #include "context.hpp"
 #include "kernels.hpp"
+#include "vorticity.hpp"
 
 namespace ds {
 
@@ -273,45 +274,54 @@ __global__ __launch_bounds__(256) void cell_accumulate_kernel(uint32_t n, const 
     if (i < n) out[i] += in[i];
 }
 
-// Force adjoint of fluid cell i from the forced-velocity adjoint (integrate_vjp_kernel,
-// operators.cu:366-379, as a gather): each face adds dt * u_bar / count to each in-domain fluid
-// cell beside it, count = number of such cells.
-__device__ __forceinline__ void force_gather(const Geo& g, const uint8_t* __restrict__ mask, const CF3& fa, uint32_t i, int x, int y,
-                                             int z, float f[3]) {
-    const int c[3]{x, y, z};
-    const uint32_t stride[3]{1u, (uint32_t)g.nx, (uint32_t)g.nx * (uint32_t)g.ny};
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const int n_a = dim_size(g, a);
-        const uint32_t lo = fidx(g, a, x, y, z);
-        const uint32_t hi = fidx(g, a, x + (a == 0), y + (a == 1), z + (a == 2));
-        const float cnt_lo = 1.0f + ((c[a] > 0 && mask[i - stride[a]] == 0) ? 1.0f : 0.0f);
-        const float cnt_hi = 1.0f + ((c[a] + 1 < n_a && mask[i + stride[a]] == 0) ? 1.0f : 0.0f);
-        f[a] = g.dt * fa.p[a][lo] / cnt_lo + g.dt * fa.p[a][hi] / cnt_hi;
-    }
-}
-
 // Integration reverse (force part), buoyancy reverse (operators.cu:326-337) and source reverse
 // (operators.cu:297-302) fused per fluid cell. Parameter adjoints: FP64 block partials.
+// Vort: the confinement force reverse runs on the same force adjoint (vorticity_force_reverse +
+// the per-cell part of normal_reverse, operators.cu:781-812), writing vort.ga / vort.t (zeros in
+// solids) and the confinement adjoint as partial set 3; the vorticity fields in vort.in come
+// from the curl pass over the step's input velocity.
+struct VortReverse {
+    VortFields in;
+    F3 ga, t;
+};
+
+template <bool Vort>
 __global__ __launch_bounds__(256) void force_source_vjp_kernel(Geo g, const uint8_t* __restrict__ mask, CF3 forced_adj,
                                                                const float* __restrict__ rho_s, const float* __restrict__ t_s,
                                                                const float* __restrict__ rho_s_adj, const float* __restrict__ t_s_adj,
                                                                Params prm, F3 e_adj, float* __restrict__ prev_rho,
                                                                float* __restrict__ prev_t, float* __restrict__ s_rho_adj,
-                                                               float* __restrict__ s_t_adj, double* __restrict__ partials) {
-    __shared__ double lds[3 * kMaxWaves];
+                                                               float* __restrict__ s_t_adj, VortReverse vort, double* __restrict__ partials) {
+    constexpr int kParts = Vort ? 4 : 3;
+    __shared__ double lds[kParts * kMaxWaves];
     const uint32_t lb = xcd_block();
     uint32_t chunk = (g.n + gridDim.x - 1) / gridDim.x;
     chunk = (chunk + 255u) & ~255u;
     const uint32_t start = min(g.n, lb * chunk), end = min(g.n, start + chunk);
     const float amb = *prm.ambient, ar = *prm.alpha_rho, at = *prm.alpha_t;
-    double p_amb = 0.0, p_ar = 0.0, p_at = 0.0;
+    const float scale = Vort ? *prm.confinement * g.h : 0.0f;
+    double p_amb = 0.0, p_ar = 0.0, p_at = 0.0, p_conf = 0.0;
     for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
-        if (mask[i] != 0) continue;
+        if (mask[i] != 0) {
+            if constexpr (Vort) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) vort.ga.p[a][i] = vort.t.p[a][i] = 0.0f;
+            }
+            continue;
+        }
         int x, y, z;
         decode_cell(g, i, x, y, z);
         float f[3];
         force_gather(g, mask, forced_adj, i, x, y, z, f);
+        if constexpr (Vort) {
+            float ga[3], t[3];
+            p_conf += confinement_reverse_at(g, vort.in, i, x, y, z, scale, f, ga, t);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                vort.ga.p[a][i] = ga[a];
+                vort.t.p[a][i] = t[a];
+            }
+        }
         if (e_adj.p[0]) {
             e_adj.p[0][i] += f[0];
             e_adj.p[1][i] += f[1];
@@ -327,12 +337,15 @@ __global__ __launch_bounds__(256) void force_source_vjp_kernel(Geo g, const uint
         if (s_rho_adj) s_rho_adj[i] += g.dt * rsa;
         if (s_t_adj) s_t_adj[i] += g.dt * tsa;
     }
-    double v[3]{p_amb, p_ar, p_at};
-    block_sum<3>(v, lds);
+    double v[kParts];
+    v[0] = p_amb;
+    v[1] = p_ar;
+    v[2] = p_at;
+    if constexpr (Vort) v[3] = p_conf;
+    block_sum<kParts>(v, lds);
     if (threadIdx.x == 0 && partials) {
-        partials[0 * gridDim.x + lb] = v[0];
-        partials[1 * gridDim.x + lb] = v[1];
-        partials[2 * gridDim.x + lb] = v[2];
+#pragma unroll
+        for (int k = 0; k < kParts; ++k) partials[k * gridDim.x + lb] = v[k];
     }
 }
 
@@ -370,18 +383,28 @@ __global__ __launch_bounds__(256) void buoyancy_vjp_kernel(Geo g, const uint8_t*
     }
 }
 
-// One block: param_adj[k] += sum of the partials of parameter k (fixed order).
-__global__ __launch_bounds__(256) void param_finalize_kernel(const double* __restrict__ partials, uint32_t G, double* amb,
-                                                             double* ar, double* at) {
-    __shared__ double lds[3 * kMaxWaves];
-    const double* parts[3]{partials, partials + G, partials + 2 * G};
-    double red[3];
-    reduce_partials<3>(parts, G, red, lds);
+// One block: param_adj[k] += sum of the partials of parameter k (fixed order), for the K first
+// parameters (ambient, density buoyancy, temperature buoyancy, confinement).
+struct ParamOut {
+    double* p[4];
+};
+template <int K>
+__global__ __launch_bounds__(256) void param_finalize_kernel(const double* __restrict__ partials, uint32_t G, ParamOut out) {
+    __shared__ double lds[K * kMaxWaves];
+    const double* parts[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) parts[k] = partials + (size_t)k * G;
+    double red[K];
+    reduce_partials<K>(parts, G, red, lds);
     if (threadIdx.x == 0) {
-        if (amb) *amb += red[0];
-        if (ar) *ar += red[1];
-        if (at) *at += red[2];
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (out.p[k]) *out.p[k] += red[k];
     }
+}
+
+ParamOut param_out(const ds_parameter_adjoint* a) {
+    return ParamOut{{a->ambient_temperature, a->density_buoyancy, a->temperature_buoyancy, a->vorticity_confinement}};
 }
 
 // source_vjp_kernel, operators.cu:297-302
@@ -477,12 +500,22 @@ void launch_cell_accumulate(ds_context* c, const float* in, float* out) {
 void launch_force_source_vjp(ds_context* c, CF3 forced_adj, const float* rho_s, const float* t_s, const float* rho_s_adj,
                              const float* t_s_adj, const Params& prm, F3 e_adj, float* prev_rho, float* prev_t, float* s_rho_adj,
                              float* s_t_adj, const ds_parameter_adjoint* param_adj) {
-    force_source_vjp_kernel<<<kPcgBlocks, kBlock, 0, c->stream>>>(c->geo, c->mask, forced_adj, rho_s, t_s, rho_s_adj, t_s_adj, prm, e_adj,
-                                                                  prev_rho, prev_t, s_rho_adj, s_t_adj, c->param_partials);
+    if (c->vorticity) {
+        require(prm.confinement != nullptr, "parameters: vorticity_confinement must be a device pointer with confinement on");
+        const VortReverse vr{VortFields{{c->vort_w[0], c->vort_w[1], c->vort_w[2]}, c->vort_mag},
+                             F3{{c->vort_ga[0], c->vort_ga[1], c->vort_ga[2]}}, F3{{c->vort_t[0], c->vort_t[1], c->vort_t[2]}}};
+        force_source_vjp_kernel<true><<<kPcgBlocks, kBlock, 0, c->stream>>>(c->geo, c->mask, forced_adj, rho_s, t_s, rho_s_adj, t_s_adj,
+                                                                            prm, e_adj, prev_rho, prev_t, s_rho_adj, s_t_adj, vr,
+                                                                            c->param_partials);
+    } else {
+        force_source_vjp_kernel<false><<<kPcgBlocks, kBlock, 0, c->stream>>>(c->geo, c->mask, forced_adj, rho_s, t_s, rho_s_adj, t_s_adj,
+                                                                             prm, e_adj, prev_rho, prev_t, s_rho_adj, s_t_adj, VortReverse{},
+                                                                             c->param_partials);
+    }
     check_launch("force_source_vjp_kernel");
-    if (param_adj && (param_adj->ambient_temperature || param_adj->density_buoyancy || param_adj->temperature_buoyancy)) {
-        param_finalize_kernel<<<1, kBlock, 0, c->stream>>>(c->param_partials, kPcgBlocks, param_adj->ambient_temperature,
-                                                           param_adj->density_buoyancy, param_adj->temperature_buoyancy);
+    if (param_adj) {
+        if (c->vorticity) param_finalize_kernel<4><<<1, kBlock, 0, c->stream>>>(c->param_partials, kPcgBlocks, param_out(param_adj));
+        else param_finalize_kernel<3><<<1, kBlock, 0, c->stream>>>(c->param_partials, kPcgBlocks, param_out(param_adj));
         check_launch("param_finalize_kernel");
     }
 }
@@ -530,8 +563,9 @@ ds_status ds_buoyancy_vjp(ds_context* c, const float* density, const float* temp
                                                                   temperature_adj, ea, c->param_partials);
         check_launch("buoyancy_vjp_kernel");
         if (param_adj) {
-            param_finalize_kernel<<<1, kBlock, 0, c->stream>>>(c->param_partials, kPcgBlocks, param_adj->ambient_temperature,
-                                                               param_adj->density_buoyancy, param_adj->temperature_buoyancy);
+            ParamOut out = param_out(param_adj);
+            out.p[3] = nullptr;
+            param_finalize_kernel<3><<<1, kBlock, 0, c->stream>>>(c->param_partials, kPcgBlocks, out);
             check_launch("param_finalize_kernel");
         }
     });

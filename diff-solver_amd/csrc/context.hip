@@ -93,8 +93,6 @@ void validate(const ds_config& cfg, uint32_t mode) {
         for (int a = 0; a < 3; ++a)
             if (!(col.size[a] > 0.0F)) bad("collider size must be positive");
     }
-    if (cfg.vorticity_confinement_enabled)
-        throw Failure{DS_ERROR_UNSUPPORTED, "vorticity confinement is not built (set vorticity_confinement_enabled = 0)"};
     if (cfg.projection_solver != DS_PROJECTION_PCG)
         throw Failure{DS_ERROR_UNSUPPORTED, "only the PCG projection solver runs on the GPU"};
     if (cfg.warm_start > DS_WARM_START_CNN) bad("warm_start out of range");
@@ -433,7 +431,17 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
             }
             c->lambda_hi = device_alloc<float>(c, n);
             c->lambda_lo = device_alloc<float>(c, n);
-            c->param_partials = device_alloc<double>(c, 3 * (size_t)kPcgBlocks);
+            c->param_partials = device_alloc<double>(c, 4 * (size_t)kPcgBlocks);
+        }
+        if (cfg.vorticity_confinement_enabled) {
+            c->vorticity = true;
+            for (int a = 0; a < 3; ++a) c->vort_w[a] = device_alloc<float>(c, n);
+            c->vort_mag = device_alloc<float>(c, n);
+            if (mode == DS_MODE_DIFFERENTIABLE)
+                for (int a = 0; a < 3; ++a) {
+                    c->vort_ga[a] = device_alloc<float>(c, n);
+                    c->vort_t[a] = device_alloc<float>(c, n);
+                }
         }
         DS_HIP(hipStreamSynchronize(c->stream));   // model.cpp:181
     });

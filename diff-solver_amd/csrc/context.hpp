@@ -193,7 +193,14 @@ struct ds_context {
     float* lambda_hi = nullptr;       // adjoint pressure solve solution (hi + lo)
     float* lambda_lo = nullptr;
     bool lambda_valid = false;
-    double* param_partials = nullptr; // 3 * kPcgBlocks
+    double* param_partials = nullptr; // 4 * kPcgBlocks
+
+    // vorticity confinement (device; allocated when vorticity_confinement_enabled)
+    bool vorticity = false;
+    float* vort_w[3]{};               // vorticity of the step's input velocity
+    float* vort_mag = nullptr;        // smoothed magnitude sqrt(|w|^2 + eps^2)
+    float* vort_ga[3]{};              // adjoint of the magnitude gradient (differentiable mode)
+    float* vort_t[3]{};               // vorticity adjoint (differentiable mode)
 
     ds::PcgWork pcg;
     ds::Profiler profiler;

@@ -6,6 +6,7 @@
 
 #include "context.hpp"
 #include "kernels.hpp"
+#include "vorticity.hpp"
 
 namespace ds {
 
@@ -35,10 +36,13 @@ __global__ __launch_bounds__(256) void buoyancy_kernel(Geo g, const uint8_t* __r
 }
 
 // source (rho and T) and buoyancy fused: one pass over the cell fields (Model::forward_step
-// steps 1-3, model.cpp:342-344).
+// steps 1-3, model.cpp:342-344). Vort: the confinement force is added in the same pass
+// (ForceAssemblyOperator::forward, operators.cpp:125-128), from the vorticity fields that the
+// curl pass wrote for the step's input velocity.
+template <bool Vort>
 __global__ __launch_bounds__(256) void source_buoyancy_kernel(Geo g, const uint8_t* __restrict__ mask, const float* __restrict__ rho,
                                                               const float* __restrict__ T, const float* __restrict__ s_rho,
-                                                              const float* __restrict__ s_t, CF3 e, Params prm,
+                                                              const float* __restrict__ s_t, CF3 e, Params prm, VortFields vort,
                                                               float* __restrict__ rho_s, float* __restrict__ t_s, F3 f) {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= g.n) return;
@@ -53,9 +57,15 @@ __global__ __launch_bounds__(256) void source_buoyancy_kernel(Geo g, const uint8
     const float ts = T[i] + g.dt * s_t[i];
     rho_s[i] = rs;
     t_s[i] = ts;
-    f.p[0][i] = e.p[0][i];
-    f.p[1][i] = e.p[1][i] + ar * rs + at * (ts - amb);
-    f.p[2][i] = e.p[2][i];
+    float fv[3]{e.p[0][i], e.p[1][i] + ar * rs + at * (ts - amb), e.p[2][i]};
+    if constexpr (Vort) {
+        int x, y, z;
+        decode_cell(g, i, x, y, z);
+        add_confinement_force(vort, vort.mag, g, *prm.confinement * g.h, i, x, y, z, fv);
+    }
+    f.p[0][i] = fv[0];
+    f.p[1][i] = fv[1];
+    f.p[2][i] = fv[2];
 }
 
 // averaged_center_force + integrate_kernel, operators.cu:339-364
@@ -246,15 +256,21 @@ void copy_pair(ds_context* c, const float* hi, const float* lo, float* out_hi, f
 
 Params params_of(const ds_parameters* p) {
     require(p && p->ambient_temperature && p->density_buoyancy && p->temperature_buoyancy, "parameters must be device pointers");
-    return Params{p->ambient_temperature, p->density_buoyancy, p->temperature_buoyancy};
+    return Params{p->ambient_temperature, p->density_buoyancy, p->temperature_buoyancy, p->vorticity_confinement};
 }
 
 CF3 collider_of(const ds_context* c) { return CF3{{c->collider_velocity[0], c->collider_velocity[1], c->collider_velocity[2]}}; }
 
 void launch_source_buoyancy(ds_context* c, const float* rho, const float* T, const float* s_rho, const float* s_t, CF3 e,
                             const Params& prm, float* rho_s, float* t_s, F3 f) {
-    source_buoyancy_kernel<<<xcd_blocks_for(c->geo.n), kBlock, 0, c->stream>>>(c->geo, c->mask, rho, T, s_rho, s_t, e, prm, rho_s,
-                                                                               t_s, f);
+    const uint32_t G = xcd_blocks_for(c->geo.n);
+    if (c->vorticity) {
+        require(prm.confinement != nullptr, "parameters: vorticity_confinement must be a device pointer with confinement on");
+        const VortFields v{{c->vort_w[0], c->vort_w[1], c->vort_w[2]}, c->vort_mag};
+        source_buoyancy_kernel<true><<<G, kBlock, 0, c->stream>>>(c->geo, c->mask, rho, T, s_rho, s_t, e, prm, v, rho_s, t_s, f);
+    } else {
+        source_buoyancy_kernel<false><<<G, kBlock, 0, c->stream>>>(c->geo, c->mask, rho, T, s_rho, s_t, e, prm, VortFields{}, rho_s, t_s, f);
+    }
     check_launch("source_buoyancy_kernel");
 }
 

@@ -16,7 +16,26 @@ struct Params {   // device pointers to one float each (Parameters, data.ixx:129
     const float* ambient;
     const float* alpha_rho;
     const float* alpha_t;
+    const float* confinement;   // null unless vorticity confinement is on
 };
+
+// Force adjoint of fluid cell i from the forced-velocity adjoint (integrate_vjp_kernel,
+// operators.cu:366-379, as a gather): each face adds dt * u_bar / count to each in-domain fluid
+// cell beside it, count = number of such cells.
+__device__ __forceinline__ void force_gather(const Geo& g, const uint8_t* __restrict__ mask, const CF3& fa, uint32_t i, int x, int y,
+                                             int z, float f[3]) {
+    const int c[3]{x, y, z};
+    const uint32_t stride[3]{1u, (uint32_t)g.nx, (uint32_t)g.nx * (uint32_t)g.ny};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const int n_a = dim_size(g, a);
+        const uint32_t lo = fidx(g, a, x, y, z);
+        const uint32_t hi = fidx(g, a, x + (a == 0), y + (a == 1), z + (a == 2));
+        const float cnt_lo = 1.0f + ((c[a] > 0 && mask[i - stride[a]] == 0) ? 1.0f : 0.0f);
+        const float cnt_hi = 1.0f + ((c[a] + 1 < n_a && mask[i + stride[a]] == 0) ? 1.0f : 0.0f);
+        f[a] = g.dt * fa.p[a][lo] / cnt_lo + g.dt * fa.p[a][hi] / cnt_hi;
+    }
+}
 
 Params params_of(const ds_parameters* p);
 CF3 collider_of(const ds_context* c);
@@ -29,5 +48,16 @@ void launch_project(ds_context* c, CF3 u, const float* ph, const float* pl, F3 o
 void launch_advect_scalars(ds_context* c, int nf, int first_field, const float* src0, const float* src1, CF3 u, float* out0,
                            float* out1);
 void copy_pair(ds_context* c, const float* hi, const float* lo, float* out_hi, float* out_lo);
+
+// Vorticity confinement (vorticity.hip). The curl pass writes c->vort_w / c->vort_mag from the
+// face velocity u; the force and reverse passes read them.
+void launch_vorticity_curl(ds_context* c, CF3 u);
+// out += the confinement force (fluid cells).
+void launch_vorticity_force(ds_context* c, const Params& prm, F3 out);
+// Reverse from a cell force adjoint fa (after launch_vorticity_curl on the same velocity):
+// u_adj += the velocity adjoint, *conf_adj += the confinement adjoint (conf_adj may be null).
+void launch_vorticity_vjp(ds_context* c, const Params& prm, CF3 fa, F3 u_adj, double* conf_adj);
+// The magnitude and curl/centring reverses, after the fused force reverse filled c->vort_ga / c->vort_t.
+void launch_vorticity_vjp_tail(ds_context* c, F3 u_adj);
 
 }  // namespace ds

@@ -110,6 +110,7 @@ ds_solve_info forward_step(ds_context* c, const ds_state* s, const ds_control* k
     F3 ua{{cache->advected_velocity[0] ? cache->advected_velocity[0] : c->advected[0],
            cache->advected_velocity[1] ? cache->advected_velocity[1] : c->advected[1],
            cache->advected_velocity[2] ? cache->advected_velocity[2] : c->advected[2]}};
+    if (c->vorticity) launch_vorticity_curl(c, CF3{{s->velocity[0], s->velocity[1], s->velocity[2]}});
     launch_source_buoyancy(c, s->density, s->temperature, k->density_source, k->temperature_source,
                            CF3{{k->external_acceleration[0], k->external_acceleration[1], k->external_acceleration[2]}}, prm,
                            cache->sourced_density, cache->sourced_temperature, f);
@@ -136,12 +137,16 @@ ds_solve_info forward_step(ds_context* c, const ds_state* s, const ds_control* k
 
 // Model::vjp_step (model.cpp:361-389): scalar advection reverse (rho and T in one pass), projection
 // reverse (adjoint PCG), constraint + velocity advection + integration reverse, buoyancy and source
-// reverse. prev_adj / control_adj / param_adj accumulate.
-ds_solve_info vjp_step(ds_context* c, const ds_parameters* p, const ds_state* next, const ds_step_cache* cache,
+// reverse, and with confinement on its reverse (vorticity recomputed from the input velocity of
+// the step, state->velocity). prev_adj / control_adj / param_adj accumulate.
+ds_solve_info vjp_step(ds_context* c, const ds_parameters* p, const ds_state* state, const ds_state* next, const ds_step_cache* cache,
                        const ds_state_adjoint* next_adj, ds_state_adjoint* prev_adj, ds_control_adjoint* control_adj,
                        const ds_parameter_adjoint* param_adj) {
     require(c->mode == DS_MODE_DIFFERENTIABLE, "vjp_step: the context was not created in differentiable mode");
     check_state(next, "vjp_step: next_state has a null field");
+    if (c->vorticity)
+        require(state && state->velocity[0] && state->velocity[1] && state->velocity[2],
+                "vjp_step: vorticity confinement needs the step's input velocity (state)");
     require(next_adj && next_adj->density && next_adj->temperature && next_adj->velocity[0] && next_adj->velocity[1] &&
                 next_adj->velocity[2],
             "vjp_step: next_state_adjoint has a null field");
@@ -184,8 +189,10 @@ ds_solve_info vjp_step(ds_context* c, const ds_parameters* p, const ds_state* ne
         sra = control_adj->density_source;
         sta = control_adj->temperature_source;
     }
+    if (c->vorticity) launch_vorticity_curl(c, CF3{{state->velocity[0], state->velocity[1], state->velocity[2]}});
     launch_force_source_vjp(c, fa, cache->sourced_density, cache->sourced_temperature, c->sourced_adj[0], c->sourced_adj[1], prm, ea,
                             prev_adj->density, prev_adj->temperature, sra, sta, param_adj);
+    if (c->vorticity) launch_vorticity_vjp_tail(c, F3{{prev_adj->velocity[0], prev_adj->velocity[1], prev_adj->velocity[2]}});
     return si;
 }
 
@@ -308,12 +315,13 @@ struct AbiModel {
     void forward_step(const AState& s, const ds_control& k, const ds_parameters& p, AState& n, ACache& cache, ACtx& x) const {
         x.last = ds::forward_step(x.c, &s.v, &k, &p, &n.v, &cache.v);
     }
-    void vjp_step(const AState&, const ds_control&, const ds_parameters& p, const AState& n, const ACache& cache, const AState& na,
+    void vjp_step(const AState& s, const ds_control&, const ds_parameters& p, const AState& n, const ACache& cache, const AState& na,
                   AState& pa, ds_control_adjoint& ca, ds_parameter_adjoint& qa, ACtx& x) const {
         const ds_state_adjoint nav = as_adjoint(na.v);
         ds_state_adjoint pav = as_adjoint(pa.v);
-        x.last = ds::vjp_step(x.c, &p, &n.v, &cache.v, &nav, &pav, ca.density_source || ca.external_acceleration[0] ? &ca : nullptr,
-                              qa.density_buoyancy || qa.ambient_temperature || qa.temperature_buoyancy ? &qa : nullptr);
+        const bool any_param = qa.density_buoyancy || qa.ambient_temperature || qa.temperature_buoyancy || qa.vorticity_confinement;
+        x.last = ds::vjp_step(x.c, &p, &s.v, &n.v, &cache.v, &nav, &pav, ca.density_source || ca.external_acceleration[0] ? &ca : nullptr,
+                              any_param ? &qa : nullptr);
     }
     static void after_forward(ACtx& x, int kind) {
         record(x.info, x.last, kind);
@@ -399,14 +407,13 @@ ds_status ds_forward_step(ds_context* c, const ds_state* state, const ds_control
 ds_status ds_vjp_step(ds_context* c, const ds_state* state, const ds_control* control, const ds_parameters* params,
                       const ds_state* next_state, const ds_step_cache* cache, const ds_state_adjoint* next_adj, ds_state_adjoint* prev_adj,
                       ds_control_adjoint* control_adj, const ds_parameter_adjoint* param_adj, ds_step_info* info) {
-    (void)state;
     (void)control;
     if (!c) {
         set_error("ds_vjp_step: null context");
         return DS_ERROR_INVALID_ARGUMENT;
     }
     ds_solve_info si{};
-    const ds_status st = guard([&] { si = vjp_step(c, params, next_state, cache, next_adj, prev_adj, control_adj, param_adj); });
+    const ds_status st = guard([&] { si = vjp_step(c, params, state, next_state, cache, next_adj, prev_adj, control_adj, param_adj); });
     if (info) info->projection = si;
     if (st != DS_OK) return st;
     if (!si.converged) {

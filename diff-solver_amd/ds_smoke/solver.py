@@ -59,6 +59,9 @@ def load_library(path: str | None = None):
         "ds_buoyancy_forward": (C.c_int32, [P, P, P, C.POINTER(P), C.POINTER(abi.Parameters), C.POINTER(P)]),
         "ds_buoyancy_vjp": (C.c_int32, [P, P, P, C.POINTER(abi.Parameters), C.POINTER(P), P, P, C.POINTER(P),
                                         C.POINTER(abi.ParameterAdjoint)]),
+        "ds_vorticity_forward": (C.c_int32, [P, C.POINTER(P), C.POINTER(abi.Parameters), C.POINTER(P)]),
+        "ds_vorticity_vjp": (C.c_int32, [P, C.POINTER(P), C.POINTER(abi.Parameters), C.POINTER(P), C.POINTER(P),
+                                         C.POINTER(abi.ParameterAdjoint)]),
         "ds_integrate_velocity_forward": (C.c_int32, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
         "ds_integrate_velocity_vjp": (C.c_int32, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
         "ds_advect_velocity_forward": (C.c_int32, [P, C.POINTER(P), C.POINTER(P)]),
@@ -353,6 +356,18 @@ class Context:
         self._check(self.lib.ds_buoyancy_vjp(self.h, _ptr(rho), _ptr(T), C.byref(self.params_view(params)), _p3(fa),
                                              _ptr(rho_adj), _ptr(T_adj), _p3(e_adj),
                                              C.byref(self.param_adjoint_view(param_adj))), "ds_buoyancy_vjp")
+
+    def vorticity_forward(self, u, params, f=None):
+        """f += the vorticity-confinement force of the face velocity u (f defaults to zeros)."""
+        f = tuple(self.zeros_cell() for _ in range(3)) if f is None else f
+        self._check(self.lib.ds_vorticity_forward(self.h, _p3(u), C.byref(self.params_view(params)), _p3(f)), "ds_vorticity_forward")
+        return f
+
+    def vorticity_vjp(self, u, params, fa, ua, param_adj=None):
+        """ua += the velocity adjoint for the cell force adjoint fa; param_adj[3] += the confinement adjoint."""
+        pa = C.byref(self.param_adjoint_view(param_adj)) if param_adj is not None else None
+        self._check(self.lib.ds_vorticity_vjp(self.h, _p3(u), C.byref(self.params_view(params)), _p3(fa), _p3(ua), pa),
+                    "ds_vorticity_vjp")
 
     def integrate_forward(self, u, f):
         out = self.zeros_faces()

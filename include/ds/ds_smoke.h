@@ -100,7 +100,7 @@ typedef struct ds_config {
     ds_scalar_boundary temperature_boundary;
     const ds_collider* colliders;           /* host array, read at context creation only */
     uint32_t collider_count;
-    uint32_t vorticity_confinement_enabled; /* must be 0: vorticity confinement is not built */
+    uint32_t vorticity_confinement_enabled; /* adds the confinement force (data.ixx:82) */
     uint32_t projection_solver;             /* DS_PROJECTION_* */
     uint32_t cg_max_iterations;             /* 0 selects 4 * (nx + ny + nz) + 1000 */
     double cg_tolerance;                    /* ||b - A x||_2 <= tol * ||b||_2, FP64 norms; 0 selects 1e-6 */
@@ -132,7 +132,8 @@ typedef struct ds_parameters {          /* each a device pointer to ONE float */
     float* vorticity_confinement;
 } ds_parameters;
 
-/* StepCache (data.ixx:189-196) without the vorticity cache. force and advected_velocity may be
+/* StepCache (data.ixx:189-196) without the vorticity cache (the reverse recomputes the vorticity
+ * from the step's input velocity, one stencil pass). force and advected_velocity may be
  * NULL (context scratch is used; the VJP does not read them). The pressure is kept as a
  * compensated FP32 pair (hi then lo, 2 * cell_count floats): pressure (may be NULL) receives
  * the converged pair of the forward step; pressure_guess (may be NULL) is read as the initial
@@ -244,6 +245,17 @@ ds_status ds_buoyancy_vjp(ds_context* c, const float* density, const float* temp
                           const ds_parameters* params, float* const force_adj[3], float* density_adj,
                           float* temperature_adj, float* external_acceleration_adj[3],
                           const ds_parameter_adjoint* param_adj);
+/* vorticity_forward (operators.cu:973-979): force += confinement * h * (N x w) on fluid cells,
+ * w = curl of the cell-centred velocity, N = normalised gradient of the smoothed |w|. Needs a
+ * context created with vorticity_confinement_enabled = 1; reads params->vorticity_confinement. */
+ds_status ds_vorticity_forward(ds_context* c, float* const velocity[3], const ds_parameters* params,
+                               float* force[3]);
+/* vorticity_vjp (operators.cu:990-1002): velocity_adj += the velocity adjoint of the force above
+ * for the cell force adjoint force_adj; param_adj->vorticity_confinement (FP64) += its
+ * confinement adjoint when param_adj is not NULL. Differentiable context with confinement on. */
+ds_status ds_vorticity_vjp(ds_context* c, float* const velocity[3], const ds_parameters* params,
+                           float* const force_adj[3], float* velocity_adj[3],
+                           const ds_parameter_adjoint* param_adj);
 /* integrate_velocity_forward (operators.cu:339-364). */
 ds_status ds_integrate_velocity_forward(ds_context* c, float* const velocity[3], float* const force[3],
                                         float* out[3]);
@@ -317,7 +329,9 @@ ds_status ds_forward_step(ds_context* c, const ds_state* state, const ds_control
                           const ds_parameters* params, ds_state* next_state, ds_step_cache* cache,
                           ds_step_info* info);
 /* Model::vjp_step (model.cpp:361-389). Reads next_state->velocity and the cache's sourced fields
- * and forced velocity. prev_adj, control_adj and param_adj are accumulated (+=). */
+ * and forced velocity; with vorticity confinement on also state->velocity (the vorticity fields
+ * are recomputed from it instead of being cached). prev_adj, control_adj and param_adj are
+ * accumulated (+=). */
 ds_status ds_vjp_step(ds_context* c, const ds_state* state, const ds_control* control,
                       const ds_parameters* params, const ds_state* next_state,
                       const ds_step_cache* cache, const ds_state_adjoint* next_adj,

@@ -63,6 +63,7 @@ int main(int argc, char** argv) {
                                                &cfg.velocity_boundary.y_max, &cfg.velocity_boundary.z_min, &cfg.velocity_boundary.z_max})
             f->mode = smoke::VelocityBoundaryMode::fixed_value;   // closed no-slip box
         cfg.cg_tolerance = 1e-6;
+        cfg.vorticity_confinement_enabled = false;   // the benchmark plume runs without confinement
         const smoke::Model model{cfg};
         smoke::ExecutionContext ctx = model.allocate_context(smoke::ExecutionMode::differentiable);
         const std::size_t cells = ctx.domain.cell_count;
