@@ -55,6 +55,12 @@ def lib():
         L.oracle_vorticity_forward.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p]
         L.oracle_vorticity_vjp.restype = None
         L.oracle_vorticity_vjp.argtypes = [C.c_void_p, C.c_void_p, C.c_double, C.c_void_p, C.c_void_p, DP]
+        for name in ("oracle_buoyancy_jvp", "oracle_advect_velocity_jvp", "oracle_constrain_jvp", "oracle_advect_scalar_jvp"):
+            getattr(L, name).restype = None
+        L.oracle_vorticity_jvp.restype = None
+        L.oracle_vorticity_jvp.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p]
+        for name in ("oracle_projection_jvp", "oracle_jvp_step", "oracle_trajectory_jvp"):
+            getattr(L, name).restype = C.c_int
         L.oracle_set_threads.restype = C.c_int
         L.oracle_set_threads.argtypes = [C.c_int]
         _lib = L
@@ -179,6 +185,42 @@ class Oracle:
         lib().oracle_vorticity_vjp(C.c_void_p(self.h), _arr3([f64(x) for x in u]), float(confinement),
                                    _arr3([f64(x) for x in fa]), _arr3(ua), _p(conf_adj))
 
+    # -- forward mode (tangents)
+    def buoyancy_jvp(self, rho, T, rho_t, T_t, e_t, params, params_t):
+        f = tuple(self.zeros_cell() for _ in range(3))
+        lib().oracle_buoyancy_jvp(C.c_void_p(self.h), _p(f64(rho)), _p(f64(T)), _p(f64(rho_t)), _p(f64(T_t)),
+                                  _arr3([f64(x) for x in e_t]), _p(f64(params)), _p(f64(params_t)), _arr3(f))
+        return f
+
+    def vorticity_jvp(self, u, u_t, confinement, confinement_t, f_t=None):
+        f_t = tuple(self.zeros_cell() for _ in range(3)) if f_t is None else tuple(f64(x).copy() for x in f_t)
+        lib().oracle_vorticity_jvp(C.c_void_p(self.h), _arr3([f64(x) for x in u]), _arr3([f64(x) for x in u_t]),
+                                   float(confinement), float(confinement_t), _arr3(f_t))
+        return f_t
+
+    def advect_velocity_jvp(self, u, u_t):
+        out = self.zeros_faces()
+        lib().oracle_advect_velocity_jvp(C.c_void_p(self.h), _arr3([f64(x) for x in u]), _arr3([f64(x) for x in u_t]), _arr3(out))
+        return out
+
+    def constrain_jvp(self, u_t):
+        out = self.zeros_faces()
+        lib().oracle_constrain_jvp(C.c_void_p(self.h), _arr3([f64(x) for x in u_t]), _arr3(out))
+        return out
+
+    def advect_scalar_jvp(self, field, src, src_t, u, u_t):
+        out = self.zeros_cell()
+        lib().oracle_advect_scalar_jvp(C.c_void_p(self.h), int(field), _p(f64(src)), _p(f64(src_t)), _arr3([f64(x) for x in u]),
+                                       _arr3([f64(x) for x in u_t]), _p(out))
+        return out
+
+    def projection_jvp(self, u_t):
+        out = self.zeros_faces()
+        info = abi.SolveInfo()
+        st = lib().oracle_projection_jvp(C.c_void_p(self.h), _arr3([f64(x) for x in u_t]), _arr3(out), C.byref(info))
+        assert st == 0, lib().oracle_last_error()
+        return out, info
+
     def integrate_forward(self, u, f):
         out = self.zeros_faces()
         lib().oracle_integrate_forward(C.c_void_p(self.h), _arr3([f64(x) for x in u]), _arr3([f64(x) for x in f]), _arr3(out))
@@ -296,6 +338,33 @@ class Oracle:
                                    _arrn(ca) if ca is not None else None, _p(param_adj), C.byref(info))
         assert st == 0, lib().oracle_last_error()
         return prev_adj, info
+
+    def jvp_step(self, state, next_state, cache, params, state_t, control_t, params_t):
+        """Model::jvp_step: the tangent of the next state (state / next_state / cache from forward_step)."""
+        fl = lambda d: [f64(d["density"]), f64(d["temperature"])] + [f64(v) for v in d["velocity"]]
+        fc = lambda d: [f64(d["density_source"]), f64(d["temperature_source"])] + [f64(v) for v in d["external_acceleration"]]
+        ch = [f64(cache["sourced_density"]), f64(cache["sourced_temperature"]), None, None, None] + \
+            [f64(v) for v in cache["forced_velocity"]] + [None, None, None, None]
+        nt = [self.zeros_cell(), self.zeros_cell()] + list(self.zeros_faces())
+        info = abi.SolveInfo()
+        st = lib().oracle_jvp_step(C.c_void_p(self.h), _arrn(fl(state)), _arrn(fl(next_state)), _arrn(ch), _p(f64(params)),
+                                   _arrn(fl(state_t)), _arrn(fc(control_t)), _p(f64(params_t)), _arrn(nt), C.byref(info))
+        assert st == 0, lib().oracle_last_error()
+        return {"density": nt[0], "temperature": nt[1], "velocity": tuple(nt[2:5])}, info
+
+    def trajectory_jvp(self, state, control, params, steps, state_t, control_t, params_t):
+        """solver::jvp in one sweep: (final state, final state tangent, TrajectoryInfo)."""
+        fl = lambda d: [f64(d["density"]), f64(d["temperature"])] + [f64(v) for v in d["velocity"]]
+        fc = lambda d: [f64(d["density_source"]), f64(d["temperature_source"])] + [f64(v) for v in d["external_acceleration"]]
+        fin = [self.zeros_cell(), self.zeros_cell()] + list(self.zeros_faces())
+        fin_t = [self.zeros_cell(), self.zeros_cell()] + list(self.zeros_faces())
+        info = abi.TrajectoryInfo()
+        st = lib().oracle_trajectory_jvp(C.c_void_p(self.h), _arrn(fl(state)), _arrn(fc(control)), _p(f64(params)), int(steps),
+                                         _arrn(fl(state_t)), _arrn(fc(control_t)), _p(f64(params_t)), _arrn(fin), _arrn(fin_t),
+                                         C.byref(info))
+        assert st == 0, lib().oracle_last_error()
+        as_state = lambda v: {"density": v[0], "temperature": v[1], "velocity": tuple(v[2:5])}
+        return as_state(fin), as_state(fin_t), info
 
     def loss_gradient(self, state, control, params, steps, k, target, want_control=False):
         s = [f64(state["density"]), f64(state["temperature"])] + [f64(v) for v in state["velocity"]]

@@ -966,6 +966,164 @@ int projection_vjp(Oracle& o, const double* const oa[3], double* const ua[3], ds
     return DS_OK;
 }
 
+// ---- forward mode: the *_jvp kernels and the forwards they reuse ---------------------------
+// The tangent of a fixed boundary value is 0: tangent samples and solves use a copy of the
+// boundary with every value zeroed (operators.cu:396-397, 516-519, 600-605, 1015-1018).
+VB zero_values(const VB& b) {
+    VB t = b;
+    for (double& v : t.values) v = 0.0;
+    return t;
+}
+SB zero_values(const SB& b) {
+    SB t = b;
+    for (double& v : t.values) v = 0.0;
+    return t;
+}
+
+V3 sample_velocity_in(const Oracle& o, const double* const u[3], V3 p, const VB& b) {
+    return {sample_face(o, u[0], 0, p, b).value, sample_face(o, u[1], 1, p, b).value, sample_face(o, u[2], 2, p, b).value};
+}
+
+// buoyancy_jvp_kernel, operators.cu:315-324. pt = parameter tangents (same order as params).
+void buoyancy_jvp(const Oracle& o, const double* rho, const double* T, const double* rho_t, const double* T_t,
+                  const double* const e_t[3], const double* params, const double* pt, double* const f_t[3]) {
+    for (i64 i = 0; i < o.n; ++i) {
+        if (o.mask[i] != 0) { f_t[0][i] = f_t[1][i] = f_t[2][i] = 0.0; continue; }
+        f_t[0][i] = e_t[0][i];
+        f_t[1][i] = e_t[1][i] + params[1] * rho_t[i] + pt[1] * rho[i] + params[2] * (T_t[i] - pt[0]) + pt[2] * (T[i] - params[0]);
+        f_t[2][i] = e_t[2][i];
+    }
+}
+
+// vorticity_jvp (operators.cu:981-988): the tangent vorticity is the curl of the centred tangent
+// velocity (the same linear kernels), magnitude_tangent_kernel (:731-735), normal_tangent_kernel
+// (:737-754) and vorticity_force_tangent_kernel (:756-779). Adds to f_t.
+void vorticity_jvp(const Oracle& o, const double* const u[3], const double* const u_t[3], double confinement,
+                   double confinement_t, double* const f_t[3]) {
+    VorticityFields v, vt;
+    vorticity_fields(o, u, v);
+    vorticity_fields(o, u_t, vt);   // vt.w = curl of the centred tangent velocity
+    std::vector<double> m_t(o.n, 0.0);
+    for (i64 i = 0; i < o.n; ++i)
+        if (o.mask[i] == 0) m_t[i] = (v.w[0][i] * vt.w[0][i] + v.w[1][i] * vt.w[1][i] + v.w[2][i] * vt.w[2][i]) / v.mag[i];
+    const double s = 0.5 / o.h;
+    auto M = [&](int x, int y, int z) { return m_t[clamped_cell(o, x, y, z)]; };
+    for (int z = 0; z < o.nz; ++z)
+        for (int y = 0; y < o.ny; ++y)
+            for (int x = 0; x < o.nx; ++x) {
+                const i64 i = cell(o, x, y, z);
+                if (o.mask[i] != 0) continue;
+                const double gt[3]{(M(x + 1, y, z) - M(x - 1, y, z)) * s, (M(x, y + 1, z) - M(x, y - 1, z)) * s,
+                                   (M(x, y, z + 1) - M(x, y, z - 1)) * s};
+                const double N[3]{v.nrm[0][i], v.nrm[1][i], v.nrm[2][i]};
+                const double proj = N[0] * gt[0] + N[1] * gt[1] + N[2] * gt[2];
+                const double Nt[3]{(gt[0] - N[0] * proj) / v.len[i], (gt[1] - N[1] * proj) / v.len[i], (gt[2] - N[2] * proj) / v.len[i]};
+                const double w[3]{v.w[0][i], v.w[1][i], v.w[2][i]};
+                const double wt[3]{vt.w[0][i], vt.w[1][i], vt.w[2][i]};
+                const double cross[3]{N[1] * w[2] - N[2] * w[1], N[2] * w[0] - N[0] * w[2], N[0] * w[1] - N[1] * w[0]};
+                const double cross_t[3]{Nt[1] * w[2] + N[1] * wt[2] - Nt[2] * w[1] - N[2] * wt[1],
+                                        Nt[2] * w[0] + N[2] * wt[0] - Nt[0] * w[2] - N[0] * wt[2],
+                                        Nt[0] * w[1] + N[0] * wt[1] - Nt[1] * w[0] - N[1] * wt[0]};
+                for (int a = 0; a < 3; ++a) f_t[a][i] += o.h * (confinement_t * cross[a] + confinement * cross_t[a]);
+            }
+}
+
+// Tangent of the RK2 velocity at the trace's midpoint and the end-point tangent (the shared part
+// of advect_velocity_jvp_kernel, operators.cu:390-413, and advect_scalar_jvp_kernel, :509-535).
+V3 trace_tangent(const Oracle& o, V3 start, const double* const u[3], const double* const u_t[3], const VB& tb, Trace& trace) {
+    const V3 v0 = sample_velocity_in(o, u, start, o.vb);
+    const V3 t0 = sample_velocity_in(o, u_t, start, tb);
+    const V3 mid{start.x - 0.5 * o.dt * v0.x, start.y - 0.5 * o.dt * v0.y, start.z - 0.5 * o.dt * v0.z};
+    const V3 mid_t{-0.5 * o.dt * t0.x, -0.5 * o.dt * t0.y, -0.5 * o.dt * t0.z};
+    const Sample s[3]{sample_face(o, u[0], 0, mid, o.vb), sample_face(o, u[1], 1, mid, o.vb), sample_face(o, u[2], 2, mid, o.vb)};
+    const V3 st = sample_velocity_in(o, u_t, mid, tb);
+    const double v1t[3]{st.x + s[0].gradient.x * mid_t.x + s[0].gradient.y * mid_t.y + s[0].gradient.z * mid_t.z,
+                        st.y + s[1].gradient.x * mid_t.x + s[1].gradient.y * mid_t.y + s[1].gradient.z * mid_t.z,
+                        st.z + s[2].gradient.x * mid_t.x + s[2].gradient.y * mid_t.y + s[2].gradient.z * mid_t.z};
+    trace = trace_rk2(o, start, u);
+    return {-trace.derivative.x * o.dt * v1t[0], -trace.derivative.y * o.dt * v1t[1], -trace.derivative.z * o.dt * v1t[2]};
+}
+
+// advect_velocity_jvp_kernel, operators.cu:390-413 (every face, like the forward).
+void advect_velocity_jvp(const Oracle& o, const double* const u[3], const double* const u_t[3], double* const out_t[3]) {
+    const VB tb = zero_values(o.vb);
+    for (int a = 0; a < 3; ++a) {
+        const int ex = extent(o, a, 0), ey = extent(o, a, 1), ez = extent(o, a, 2);
+#pragma omp parallel for
+        for (int z = 0; z < ez; ++z)
+            for (int y = 0; y < ey; ++y)
+                for (int x = 0; x < ex; ++x) {
+                    Trace t;
+                    const V3 pt = trace_tangent(o, face_position(o, a, x, y, z), u, u_t, tb, t);
+                    const Sample src = sample_face(o, u[a], a, t.position, o.vb);
+                    out_t[a][idx3(x, y, z, ex, ey)] = sample_face(o, u_t[a], a, t.position, tb).value + src.gradient.x * pt.x +
+                                                      src.gradient.y * pt.y + src.gradient.z * pt.z;
+                }
+    }
+}
+
+// constrain_velocity_jvp, operators.cu:599-606: the constraint with zero boundary values and no
+// collider velocity.
+void constrain_jvp(const Oracle& o, const double* const u_t[3], double* const out_t[3]) {
+    for (int a = 0; a < 3; ++a) {
+        const int ex = extent(o, a, 0), ey = extent(o, a, 1), ez = extent(o, a, 2);
+        for (int z = 0; z < ez; ++z)
+            for (int y = 0; y < ey; ++y)
+                for (int x = 0; x < ex; ++x) {
+                    const i64 fi = idx3(x, y, z, ex, ey);
+                    double bv = 0.0;
+                    if (constrained_boundary(o, a, x, y, z, &bv) || adjacent_solid(o, a, x, y, z)) { out_t[a][fi] = 0.0; continue; }
+                    const int coord = a == 0 ? x : a == 1 ? y : z;
+                    if (vperiodic(o.vb, a) && coord == size_of(o, a)) {
+                        out_t[a][fi] = u_t[a][idx3(a == 0 ? 0 : x, a == 1 ? 0 : y, a == 2 ? 0 : z, ex, ey)];
+                        continue;
+                    }
+                    out_t[a][fi] = u_t[a][fi];
+                }
+    }
+}
+
+// advect_scalar_jvp_kernel, operators.cu:509-535 (solid cells: 0).
+void advect_scalar_jvp(const Oracle& o, int field, const double* src, const double* src_t, const double* const u[3],
+                       const double* const u_t[3], double* out_t) {
+    const SB& b = scalar_bc(o, field);
+    const SB sbt = zero_values(b);
+    const VB tb = zero_values(o.vb);
+#pragma omp parallel for
+    for (int z = 0; z < o.nz; ++z)
+        for (int y = 0; y < o.ny; ++y)
+            for (int x = 0; x < o.nx; ++x) {
+                const i64 i = cell(o, x, y, z);
+                if (o.mask[i] != 0) { out_t[i] = 0.0; continue; }
+                Trace t;
+                const V3 pt = trace_tangent(o, cell_position(o, x, y, z), u, u_t, tb, t);
+                const Sample s = sample_scalar(o, src, t.position, b);
+                out_t[i] = sample_scalar(o, src_t, t.position, sbt).value + s.gradient.x * pt.x + s.gradient.y * pt.y + s.gradient.z * pt.z;
+            }
+}
+
+// projection_jvp, operators.cu:1015-1018: the projection of the tangent with zero pressure
+// boundary values (PCG mode: the right-hand side without the fixed-face term; RBGS mode: the
+// sweeps with zeroed boundary values).
+int projection_jvp(Oracle& o, const double* const u_t[3], double* const out_t[3], ds_solve_info* info) {
+    std::vector<double> rhs(o.n), p(o.n, 0.0);
+    pressure_rhs(o, u_t, rhs.data());
+    if (o.projection_solver == DS_PROJECTION_RBGS) {
+        const SB saved = o.pb;
+        o.pb = zero_values(o.pb);
+        rbgs(o, rhs.data(), p.data(), o.rbgs_iterations);
+        o.pb = saved;
+        if (info) *info = ds_solve_info{o.rbgs_iterations, 1, 0, 0, 0.0, 0.0, 0.0};
+    } else {
+        for (i64 i = 0; i < o.n; ++i)
+            if (!o.code[i]) rhs[i] = 0.0;
+        const int st = pcg(o, rhs.data(), nullptr, p.data(), o.tolerance, o.max_iterations, info);
+        if (st != DS_OK) return st;
+    }
+    project(o, u_t, p.data(), out_t);
+    return DS_OK;
+}
+
 // ---- model steps (model.cpp:341-389) --------------------------------------------------------
 // state/next: {rho, T, ux, uy, uz}; control: {s_rho, s_T, ex, ey, ez}; params: {ambient,
 // alpha_rho, alpha_T, confinement}; cache: {rho_s, T_s, fx, fy, fz, u*x, u*y, u*z, u^x, u^y, u^z,
@@ -1082,6 +1240,51 @@ int vjp_step(Oracle& o, const double* const state[5], const double* const next[5
     return DS_OK;
 }
 
+// Model::jvp_step, model.cpp:351-359. state_t / control_t / params_t are the tangents; state,
+// next and cache come from forward_step on the same inputs (cache: sourced fields and forced
+// velocity). Writes next_t.
+int jvp_step(Oracle& o, const double* const state[5], const double* const next[5], const double* const cache[12],
+             const double* params, const double* const state_t[5], const double* const control_t[5], const double* params_t,
+             double* const next_t[5], ds_solve_info* info) {
+    const i64 n = o.n;
+    std::vector<double> rs_t(n), ts_t(n), f_t[3], us_t[3], raw_t[3], ua_t[3];
+    for (int a = 0; a < 3; ++a) {
+        f_t[a].assign(n, 0.0);
+        us_t[a].assign(o.nf[a], 0.0);
+        raw_t[a].assign(o.nf[a], 0.0);
+        ua_t[a].assign(o.nf[a], 0.0);
+    }
+    source_forward(o, state_t[0], control_t[0], rs_t.data());   // source_jvp = the forward on tangents
+    source_forward(o, state_t[1], control_t[1], ts_t.data());
+    double* fp[3]{f_t[0].data(), f_t[1].data(), f_t[2].data()};
+    const double* e_t[3]{control_t[2], control_t[3], control_t[4]};
+    buoyancy_jvp(o, cache[0], cache[1], rs_t.data(), ts_t.data(), e_t, params, params_t, fp);
+    const double* ut[3]{state_t[2], state_t[3], state_t[4]};
+    if (o.vorticity) {
+        const double* u[3]{state[2], state[3], state[4]};
+        vorticity_jvp(o, u, ut, params[3], params_t[3], fp);
+    }
+    const double* fc[3]{fp[0], fp[1], fp[2]};
+    double* usp[3]{us_t[0].data(), us_t[1].data(), us_t[2].data()};
+    integrate_forward(o, ut, fc, usp);   // integrate_velocity_jvp = the forward on tangents
+    const double* us[3]{cache[5], cache[6], cache[7]};
+    const double* usc[3]{usp[0], usp[1], usp[2]};
+    double* rawp[3]{raw_t[0].data(), raw_t[1].data(), raw_t[2].data()};
+    advect_velocity_jvp(o, us, usc, rawp);
+    const double* rawc[3]{rawp[0], rawp[1], rawp[2]};
+    double* uap[3]{ua_t[0].data(), ua_t[1].data(), ua_t[2].data()};
+    constrain_jvp(o, rawc, uap);
+    const double* uac[3]{uap[0], uap[1], uap[2]};
+    double* nv_t[3]{next_t[2], next_t[3], next_t[4]};
+    const int st = projection_jvp(o, uac, nv_t, info);
+    if (st != DS_OK) return st;
+    const double* nv[3]{next[2], next[3], next[4]};
+    const double* nvt[3]{next_t[2], next_t[3], next_t[4]};
+    advect_scalar_jvp(o, 0, cache[0], rs_t.data(), nv, nvt, next_t[0]);
+    advect_scalar_jvp(o, 1, cache[1], ts_t.data(), nv, nvt, next_t[1]);
+    return DS_OK;
+}
+
 void record(ds_trajectory_info* ti, const ds_solve_info& si, int kind) {
     if (!ti) return;
     if (kind == 0) { ti->forward_solves++; ti->forward_iterations += si.iterations; }
@@ -1095,6 +1298,67 @@ void record(ds_trajectory_info* ti, const ds_solve_info& si, int kind) {
 // solver::vjp (solver.ixx:115-150) with a checkpoint every k steps: the forward pass keeps
 // states 0, k, 2k, ...; the reverse sweep recomputes one segment's states and caches from its
 // checkpoint, then runs vjp_step backwards over it. The loss is 0.5 ||rho_T - target||^2.
+// solver::jvp (solver.ixx:83-113) in one forward sweep: each step runs forward_step and then
+// jvp_step on its cache, so no trajectory is stored. The control and its tangent are the same at
+// every step. Writes the final state and the final state tangent.
+int trajectory_jvp(Oracle& o, const double* const init[5], const double* const control[5], const double* params,
+                   std::uint32_t steps, const double* const init_t[5], const double* const control_t[5], const double* params_t,
+                   double* const final_state[5], double* const final_t[5], ds_trajectory_info* ti) {
+    const i64 n = o.n;
+    auto alloc_state = [&]() {
+        std::vector<std::vector<double>> s(5);
+        s[0].assign(n, 0.0);
+        s[1].assign(n, 0.0);
+        for (int a = 0; a < 3; ++a) s[2 + a].assign(o.nf[a], 0.0);
+        return s;
+    };
+    auto ptrs = [](std::vector<std::vector<double>>& s, double** out) {
+        for (int i = 0; i < (int)s.size(); ++i) out[i] = s[i].data();
+    };
+    auto cur = alloc_state(), cur_t = alloc_state();
+    for (int i = 0; i < 5; ++i) {
+        std::copy(init[i], init[i] + cur[i].size(), cur[i].begin());
+        std::copy(init_t[i], init_t[i] + cur_t[i].size(), cur_t[i].begin());
+    }
+    if (ti) {
+        *ti = ds_trajectory_info{};
+        ti->steps = steps;
+    }
+    std::vector<double> c_rs(n), c_ts(n), c_us[3];
+    for (int a = 0; a < 3; ++a) c_us[a].assign(o.nf[a], 0.0);
+    for (std::uint32_t s = 0; s < steps; ++s) {
+        auto nxt = alloc_state(), nxt_t = alloc_state();
+        double *sp[5], *np[5], *tp[5], *ntp[5];
+        ptrs(cur, sp);
+        ptrs(nxt, np);
+        ptrs(cur_t, tp);
+        ptrs(nxt_t, ntp);
+        double* cache[12]{};
+        cache[0] = c_rs.data();
+        cache[1] = c_ts.data();
+        for (int a = 0; a < 3; ++a) cache[5 + a] = c_us[a].data();
+        ds_solve_info si{};
+        const double* spc[5]{sp[0], sp[1], sp[2], sp[3], sp[4]};
+        int st = forward_step(o, spc, control, params, np, cache, nullptr, &si);
+        if (st != DS_OK) return st;
+        record(ti, si, 0);
+        const double* cc[12]{};
+        for (int i = 0; i < 12; ++i) cc[i] = cache[i];
+        const double* npc[5]{np[0], np[1], np[2], np[3], np[4]};
+        const double* tpc[5]{tp[0], tp[1], tp[2], tp[3], tp[4]};
+        st = jvp_step(o, spc, npc, cc, params, tpc, control_t, params_t, ntp, &si);
+        if (st != DS_OK) return st;
+        record(ti, si, 2);   // the linearised (tangent) solves are reported as the adjoint ones
+        cur = std::move(nxt);
+        cur_t = std::move(nxt_t);
+    }
+    for (int i = 0; i < 5; ++i) {
+        std::copy(cur[i].begin(), cur[i].end(), final_state[i]);
+        std::copy(cur_t[i].begin(), cur_t[i].end(), final_t[i]);
+    }
+    return DS_OK;
+}
+
 int loss_gradient(Oracle& o, const double* const init[5], const double* const control[5], const double* params,
                   std::uint32_t steps, std::uint32_t k, const double* target, double* const final_state[5],
                   double* loss, double* const init_adj[5], double* const control_adj[5], double* param_adj,
@@ -1506,6 +1770,39 @@ int oracle_vjp_step(void* h, const double* const state[5], const double* const n
                     const double* params, const double* const next_adj[5], double* const prev_adj[5],
                     double* const control_adj[5], double* param_adj, ds_solve_info* info) {
     return vjp_step(*static_cast<Oracle*>(h), state, next, cache, params, next_adj, prev_adj, control_adj, param_adj, info);
+}
+void oracle_buoyancy_jvp(void* h, const double* rho, const double* T, const double* rho_t, const double* T_t,
+                         const double* const e_t[3], const double* params, const double* pt, double* const f_t[3]) {
+    buoyancy_jvp(*static_cast<Oracle*>(h), rho, T, rho_t, T_t, e_t, params, pt, f_t);
+}
+void oracle_vorticity_jvp(void* h, const double* const u[3], const double* const u_t[3], double confinement, double confinement_t,
+                          double* const f_t[3]) {
+    vorticity_jvp(*static_cast<Oracle*>(h), u, u_t, confinement, confinement_t, f_t);
+}
+void oracle_advect_velocity_jvp(void* h, const double* const u[3], const double* const u_t[3], double* const out_t[3]) {
+    advect_velocity_jvp(*static_cast<Oracle*>(h), u, u_t, out_t);
+}
+void oracle_constrain_jvp(void* h, const double* const u_t[3], double* const out_t[3]) {
+    constrain_jvp(*static_cast<Oracle*>(h), u_t, out_t);
+}
+void oracle_advect_scalar_jvp(void* h, int field, const double* src, const double* src_t, const double* const u[3],
+                              const double* const u_t[3], double* out_t) {
+    advect_scalar_jvp(*static_cast<Oracle*>(h), field, src, src_t, u, u_t, out_t);
+}
+int oracle_projection_jvp(void* h, const double* const u_t[3], double* const out_t[3], ds_solve_info* info) {
+    return projection_jvp(*static_cast<Oracle*>(h), u_t, out_t, info);
+}
+int oracle_jvp_step(void* h, const double* const state[5], const double* const next[5], const double* const cache[12],
+                    const double* params, const double* const state_t[5], const double* const control_t[5],
+                    const double* params_t, double* const next_t[5], ds_solve_info* info) {
+    return jvp_step(*static_cast<Oracle*>(h), state, next, cache, params, state_t, control_t, params_t, next_t, info);
+}
+int oracle_trajectory_jvp(void* h, const double* const init[5], const double* const control[5], const double* params,
+                          std::uint32_t steps, const double* const init_t[5], const double* const control_t[5],
+                          const double* params_t, double* const final_state[5], double* const final_t[5],
+                          ds_trajectory_info* info) {
+    return trajectory_jvp(*static_cast<Oracle*>(h), init, control, params, steps, init_t, control_t, params_t, final_state,
+                          final_t, info);
 }
 int oracle_loss_gradient(void* h, const double* const init[5], const double* const control[5], const double* params,
                          std::uint32_t steps, std::uint32_t k, const double* target, double* const final_state[5],
