@@ -1,7 +1,8 @@
 // xayah/solver.hpp — generic trajectory driver, shaped like the reference's xayah.solver module
-// (solvers/solver.ixx:5-150): TapeMode, Trajectory, GradientResult, simulate and vjp over a
-// model satisfying ForwardModel / VjpModel, plus a checkpoint-every-k reverse sweep that the
-// reference does not have (its recompute mode keeps every state, solver.ixx:61-79,130-141).
+// (solvers/solver.ixx:5-150): TapeMode, Trajectory, GradientResult, simulate, jvp and vjp over a
+// model satisfying ForwardModel / JvpModel / VjpModel, plus a checkpoint-every-k reverse sweep
+// and a one-sweep forward mode that the reference does not have (its recompute mode keeps every
+// state, solver.ixx:61-79,130-141; its jvp needs the stored trajectory, :83-113).
 //
 // Checkpointed sweep: the forward pass keeps states 0, k, 2k, ... and (optionally) every
 // step's converged pressure pair. The reverse pass recomputes one segment of k steps from its
@@ -62,6 +63,21 @@ concept VjpModel = ForwardModel<Model, Context, State, Control, Parameters> &&
                        { m.copy_state_adjoint(na, pa, c) } -> std::same_as<void>;
                        { m.accumulate_state_adjoint(na, pa, c) } -> std::same_as<void>;
                        { m.vjp_step(s, k, p, n, cache, na, pa, ca, qa, c) } -> std::same_as<void>;
+                   };
+
+template <typename Model, typename Context, typename State, typename Control, typename Parameters>
+concept JvpModel = ForwardModel<Model, Context, State, Control, Parameters> &&
+                   requires(const Model& m, Context& c, const State& s, const Control& k, const Parameters& p, const State& n,
+                            const decltype(std::declval<const Model&>().allocate_step_cache(c))& cache,
+                            const decltype(std::declval<const Model&>().allocate_state_tangent(c))& st,
+                            decltype(std::declval<const Model&>().allocate_state_tangent(c))& nt,
+                            const decltype(std::declval<const Model&>().allocate_control_tangent(c))& kt,
+                            const decltype(std::declval<const Model&>().allocate_parameter_tangent(c))& pt) {
+                       m.allocate_state_tangent(c);
+                       m.allocate_control_tangent(c);
+                       m.allocate_parameter_tangent(c);
+                       { m.copy_state_tangent(st, nt, c) } -> std::same_as<void>;
+                       { m.jvp_step(s, k, p, n, cache, st, kt, pt, nt, c) } -> std::same_as<void>;
                    };
 
 // solver::simulate, solver.ixx:53-81.
@@ -125,6 +141,67 @@ template <typename Model, typename Context, typename State, typename StepCache, 
     }
     r.initial_state = std::move(running);
     return r;
+}
+
+// solver::jvp, solver.ixx:83-113: the state tangent after every step of a stored trajectory
+// (element 0 is a copy of the initial tangent).
+template <typename Model, typename Context, typename State, typename StepCache, typename Control, typename Parameters, typename StateTangent,
+          typename ControlTangent, typename ParameterTangent>
+    requires JvpModel<Model, Context, State, Control, Parameters>
+[[nodiscard]] auto jvp(const Model& model, Context& context, const Trajectory<State, StepCache>& trajectory, std::span<const Control> controls,
+                       const Parameters& parameters, const StateTangent& initial_state_tangent,
+                       std::span<const ControlTangent> control_tangents, const ParameterTangent& parameter_tangent) -> std::vector<StateTangent> {
+    std::vector<StateTangent> result;
+    result.reserve(trajectory.states.size());
+    StateTangent first = model.allocate_state_tangent(context);
+    model.copy_state_tangent(initial_state_tangent, first, context);
+    result.push_back(std::move(first));
+    std::optional<State> recomputed_state;
+    std::optional<StepCache> recomputed_cache;
+    if (trajectory.tape_mode == TapeMode::recompute_step_cache && !controls.empty()) {
+        recomputed_state.emplace(model.allocate_state(context));
+        recomputed_cache.emplace(model.allocate_step_cache(context));
+    }
+    for (std::size_t step = 0; step < controls.size(); ++step) {
+        const StepCache* cache;
+        if (trajectory.tape_mode == TapeMode::store_all) cache = &trajectory.step_caches[step];
+        else {
+            model.forward_step(trajectory.states[step], controls[step], parameters, *recomputed_state, *recomputed_cache, context);
+            cache = &*recomputed_cache;
+        }
+        StateTangent next = model.allocate_state_tangent(context);
+        model.jvp_step(trajectory.states[step], controls[step], parameters, trajectory.states[step + 1], *cache, result.back(),
+                       control_tangents[step], parameter_tangent, next, context);
+        result.push_back(std::move(next));
+    }
+    return result;
+}
+
+// Forward mode in one sweep (new): each step runs forward_step and then jvp_step on that step's
+// cache, so nothing is stored (two states, two tangents, one cache). Returns (state_T, its
+// tangent). control(step) / control_tangent(step) return a step's control and its tangent.
+// Optional statistics hooks: after_forward(c, 0) and after_jvp(c).
+template <typename Model, typename Context, typename State, typename Control, typename Parameters, typename StateTangent,
+          typename ControlTangent, typename ParameterTangent>
+    requires JvpModel<Model, Context, State, Control, Parameters>
+auto jvp_streaming(const Model& model, Context& context, const State& initial_state,
+                   const std::function<const Control&(std::uint32_t)>& control, const Parameters& parameters, std::uint32_t steps,
+                   const StateTangent& initial_tangent, const std::function<const ControlTangent&(std::uint32_t)>& control_tangent,
+                   const ParameterTangent& parameter_tangent) -> std::pair<State, StateTangent> {
+    auto cache = model.allocate_step_cache(context);
+    State a = model.allocate_state(context), b = model.allocate_state(context);
+    StateTangent at = model.allocate_state_tangent(context), bt = model.allocate_state_tangent(context);
+    model.copy_state(initial_state, a, context);
+    model.copy_state_tangent(initial_tangent, at, context);
+    for (std::uint32_t s = 0; s < steps; ++s) {
+        model.forward_step(a, control(s), parameters, b, cache, context);
+        if constexpr (requires { model.after_forward(context, 0); }) model.after_forward(context, 0);
+        model.jvp_step(a, control(s), parameters, b, cache, at, control_tangent(s), parameter_tangent, bt, context);
+        if constexpr (requires { model.after_jvp(context); }) model.after_jvp(context);
+        std::swap(a, b);
+        std::swap(at, bt);
+    }
+    return {std::move(a), std::move(at)};
 }
 
 // ---- checkpoint-every-k (new) ---------------------------------------------------------------

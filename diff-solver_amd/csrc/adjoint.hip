@@ -158,15 +158,6 @@ __global__ __launch_bounds__(256) void advect_velocity_vjp_kernel(Geo g, const V
     }
 }
 
-template <int A>
-__device__ __forceinline__ bool adjacent_solid(const Geo& g, const uint8_t* __restrict__ mask, int x, int y, int z) {
-    const int coord = A == 0 ? x : A == 1 ? y : z;
-    const int n_a = dim_size(g, A);
-    if (coord < n_a && mask[cidx(g, x, y, z)] != 0) return true;
-    if (coord > 0 && mask[cidx(g, x - (A == 0), y - (A == 1), z - (A == 2))] != 0) return true;
-    return false;
-}
-
 // constrained_boundary, operators.cu:452-458
 template <int A>
 __device__ __forceinline__ bool constrained_boundary(const Geo& g, const VBound& b, int coord) {

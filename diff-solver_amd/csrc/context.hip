@@ -340,14 +340,26 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
         c->replacement_interval = cfg.residual_replacement_interval ? cfg.residual_replacement_interval : 50u;
 
         {
-            VBound* v = device_alloc<VBound>(c, 1);
-            SBound* sb = device_alloc<SBound>(c, 2);
+            c->vb_zero = c->vb;
+            c->db_zero = c->db;
+            c->tb_zero = c->tb;
+            for (float& v : c->vb_zero.values) v = 0.0f;
+            for (float& v : c->db_zero.values) v = 0.0f;
+            for (float& v : c->tb_zero.values) v = 0.0f;
+            VBound* v = device_alloc<VBound>(c, 2);
+            SBound* sb = device_alloc<SBound>(c, 4);
             DS_HIP(hipMemcpyAsync(v, &c->vb, sizeof(VBound), hipMemcpyHostToDevice, c->stream));
+            DS_HIP(hipMemcpyAsync(v + 1, &c->vb_zero, sizeof(VBound), hipMemcpyHostToDevice, c->stream));
             DS_HIP(hipMemcpyAsync(sb, &c->db, sizeof(SBound), hipMemcpyHostToDevice, c->stream));
             DS_HIP(hipMemcpyAsync(sb + 1, &c->tb, sizeof(SBound), hipMemcpyHostToDevice, c->stream));
+            DS_HIP(hipMemcpyAsync(sb + 2, &c->db_zero, sizeof(SBound), hipMemcpyHostToDevice, c->stream));
+            DS_HIP(hipMemcpyAsync(sb + 3, &c->tb_zero, sizeof(SBound), hipMemcpyHostToDevice, c->stream));
             c->d_vb = v;
+            c->d_vb_zero = v + 1;
             c->d_db = sb;
             c->d_tb = sb + 1;
+            c->d_db_zero = sb + 2;
+            c->d_tb_zero = sb + 3;
         }
         const HostDomain d = build_domain(cfg);
         c->anchor = d.anchor;
@@ -432,16 +444,27 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
             c->lambda_hi = device_alloc<float>(c, n);
             c->lambda_lo = device_alloc<float>(c, n);
             c->param_partials = device_alloc<double>(c, 4 * (size_t)kPcgBlocks);
+            for (int k = 0; k < 2; ++k) c->sourced_t[k] = device_alloc<float>(c, n);
+            for (int a = 0; a < 3; ++a) {
+                c->force_t[a] = device_alloc<float>(c, n);
+                c->forced_t[a] = device_alloc<float>(c, g.nf[a]);
+                c->advected_t[a] = device_alloc<float>(c, g.nf[a]);
+            }
+            c->tangent_hi = device_alloc<float>(c, n);
+            c->tangent_lo = device_alloc<float>(c, n);
         }
         if (cfg.vorticity_confinement_enabled) {
             c->vorticity = true;
             for (int a = 0; a < 3; ++a) c->vort_w[a] = device_alloc<float>(c, n);
             c->vort_mag = device_alloc<float>(c, n);
-            if (mode == DS_MODE_DIFFERENTIABLE)
+            if (mode == DS_MODE_DIFFERENTIABLE) {
                 for (int a = 0; a < 3; ++a) {
                     c->vort_ga[a] = device_alloc<float>(c, n);
                     c->vort_t[a] = device_alloc<float>(c, n);
+                    c->vort_wt[a] = device_alloc<float>(c, n);
                 }
+                c->vort_mt = device_alloc<float>(c, n);
+            }
         }
         DS_HIP(hipStreamSynchronize(c->stream));   // model.cpp:181
     });

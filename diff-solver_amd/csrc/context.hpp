@@ -195,12 +195,29 @@ struct ds_context {
     bool lambda_valid = false;
     double* param_partials = nullptr; // 4 * kPcgBlocks
 
+    // forward-mode (tangent) scratch (device, differentiable mode)
+    float* sourced_t[2]{};            // tangents of rho_s, T_s
+    float* force_t[3]{};              // force tangent
+    float* forced_t[3]{};             // tangent of the forced velocity u*
+    float* advected_t[3]{};           // tangent of the advected, constrained velocity
+    float* tangent_hi = nullptr;      // pressure tangent (hi + lo)
+    float* tangent_lo = nullptr;
+    bool tangent_valid = false;
+    // boundaries with every fixed value zeroed: the tangent of a constant is 0
+    ds::VBound vb_zero{};
+    ds::SBound db_zero{}, tb_zero{};
+    const ds::VBound* d_vb_zero = nullptr;
+    const ds::SBound* d_db_zero = nullptr;
+    const ds::SBound* d_tb_zero = nullptr;
+
     // vorticity confinement (device; allocated when vorticity_confinement_enabled)
     bool vorticity = false;
     float* vort_w[3]{};               // vorticity of the step's input velocity
     float* vort_mag = nullptr;        // smoothed magnitude sqrt(|w|^2 + eps^2)
     float* vort_ga[3]{};              // adjoint of the magnitude gradient (differentiable mode)
     float* vort_t[3]{};               // vorticity adjoint (differentiable mode)
+    float* vort_wt[3]{};              // vorticity tangent (differentiable mode)
+    float* vort_mt = nullptr;         // magnitude tangent (differentiable mode)
 
     ds::PcgWork pcg;
     ds::Profiler profiler;
@@ -227,7 +244,8 @@ void fill_zero_d(ds_context* c, double* p, uint64_t count);
 ds_solve_info pcg_solve(ds_context* c, float* x_hi, float* x_lo, const float* x0, double tol, uint32_t max_it);
 
 // Solves A x = b with b built from `u` (kind 1: forward divergence of the advected velocity;
-// kind 2: adjoint, u = projected-velocity adjoint plus `extra`) or given in c->pcg.b (kind 0).
+// kind 2: adjoint, u = projected-velocity adjoint plus `extra`; kind 3: tangent, the forward
+// divergence without the fixed-face term) or given in c->pcg.b (kind 0).
 ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], const float* const extra[3], float* xh,
                              float* xl, const float* x0h, const float* x0l, double tol, uint32_t max_it);
 uint32_t pcg_periodic_bits(const ds_context* c);

@@ -91,40 +91,6 @@ __global__ __launch_bounds__(256) void integrate_kernel(Geo g, const uint8_t* __
     });
 }
 
-// adjacent_solid, operators.cu:441-450
-template <int A>
-__device__ __forceinline__ bool adjacent_solid(const Geo& g, const uint8_t* __restrict__ mask, int x, int y, int z) {
-    const int coord = A == 0 ? x : A == 1 ? y : z;
-    const int n_a = dim_size(g, A);
-    if (coord < n_a && mask[cidx(g, x, y, z)] != 0) return true;
-    if (coord > 0 && mask[cidx(g, x - (A == 0), y - (A == 1), z - (A == 2))] != 0) return true;
-    return false;
-}
-
-// Result of the constraint of face (A, x, y, z) (constrain_velocity_forward_kernel,
-// operators.cu:466-481): 0 = keep the advected value, 1 = fixed value in `value`,
-// 2 = periodic duplicate (use face 0 of the axis).
-template <int A>
-__device__ __forceinline__ int constraint(const Geo& g, const VBound& b, const uint8_t* __restrict__ mask, const CF3& collider,
-                                          bool has_solids, int x, int y, int z, uint32_t local, float& value) {
-    const int coord = A == 0 ? x : A == 1 ? y : z;
-    const int n_a = dim_size(g, A);
-    if (coord == 0 || coord == n_a) {
-        const bool hi = coord == n_a;
-        const uint32_t mode = hi ? b.modes[2 * A + 1] : b.modes[2 * A];
-        if (mode == 0u || mode == 2u) {
-            value = hi ? b.values[(2 * A + 1) * 3 + A] : b.values[(2 * A) * 3 + A];
-            return 1;
-        }
-    }
-    if (has_solids && adjacent_solid<A>(g, mask, x, y, z)) {
-        value = collider.p[A] ? collider.p[A][local] : 0.0f;
-        return 1;
-    }
-    if (vperiodic(b, A) && coord == n_a) return 2;
-    return 0;
-}
-
 // advect_velocity_forward_kernel (operators.cu:381-388), optionally with the constraint fused:
 // constrained faces skip the trace; a periodic duplicate face traces face 0's position.
 template <bool Constrain, bool HasSolids>
@@ -293,6 +259,12 @@ void launch_advect_velocity(ds_context* c, CF3 u, F3 out, bool constrain) {
     check_launch("advect_velocity_kernel");
 }
 
+void launch_constrain(ds_context* c, const VBound* bp, CF3 collider, CF3 in, F3 out) {
+    constrain_kernel<<<xcd_blocks_for(c->geo.foff[3]), kBlock, 0, c->stream>>>(c->geo, bp, c->mask, c->has_solids ? 1u : 0u, collider, in,
+                                                                                out);
+    check_launch("constrain_kernel");
+}
+
 void launch_project(ds_context* c, CF3 u, const float* ph, const float* pl, F3 out) {
     project_kernel<<<xcd_blocks_for(c->geo.foff[3]), kBlock, 0, c->stream>>>(c->geo, c->mask, u, ph, pl, out);
     check_launch("project_kernel");
@@ -381,10 +353,7 @@ ds_status ds_constrain_velocity_forward(ds_context* c, float* const velocity[3],
         return DS_ERROR_INVALID_ARGUMENT;
     }
     return guard([&] {
-        constrain_kernel<<<xcd_blocks_for(c->geo.foff[3]), kBlock, 0, c->stream>>>(
-            c->geo, c->d_vb, c->mask, c->has_solids ? 1u : 0u, collider_of(c), CF3{{velocity[0], velocity[1], velocity[2]}},
-            F3{{out[0], out[1], out[2]}});
-        check_launch("constrain_kernel");
+        launch_constrain(c, c->d_vb, collider_of(c), CF3{{velocity[0], velocity[1], velocity[2]}}, F3{{out[0], out[1], out[2]}});
     });
 }
 

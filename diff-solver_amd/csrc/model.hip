@@ -315,6 +315,19 @@ struct AbiModel {
     void forward_step(const AState& s, const ds_control& k, const ds_parameters& p, AState& n, ACache& cache, ACtx& x) const {
         x.last = ds::forward_step(x.c, &s.v, &k, &p, &n.v, &cache.v);
     }
+    // forward mode: tangents have the layout of states (ds_state_tangent), controls and parameters
+    AState allocate_state_tangent(ACtx& x) const { return allocate_state(x); }
+    ds_control allocate_control_tangent(ACtx&) const { return {}; }
+    ds_parameters allocate_parameter_tangent(ACtx&) const { return {}; }
+    void copy_state_tangent(const AState& s, AState& d, ACtx& x) const { copy_state(s, d, x); }
+    void jvp_step(const AState& s, const ds_control&, const ds_parameters& p, const AState& n, const ACache& cache, const AState& st,
+                  const ds_control& kt, const ds_parameters& pt, AState& nt, ACtx& x) const {
+        x.last = ds::jvp_step(x.c, &p, &pt, &s.v, &n.v, &cache.v, &st.v, &kt, &nt.v);
+    }
+    static void after_jvp(ACtx& x) {
+        record(x.info, x.last, 2);
+        if (!x.last.converged) x.soft = std::isfinite(x.last.final_residual) ? DS_ERROR_NOT_CONVERGED : DS_ERROR_NON_FINITE;
+    }
     void vjp_step(const AState& s, const ds_control&, const ds_parameters& p, const AState& n, const ACache& cache, const AState& na,
                   AState& pa, ds_control_adjoint& ca, ds_parameter_adjoint& qa, ACtx& x) const {
         const ds_state_adjoint nav = as_adjoint(na.v);
@@ -421,6 +434,48 @@ ds_status ds_vjp_step(ds_context* c, const ds_state* state, const ds_control* co
         return std::isfinite(si.final_residual) ? DS_ERROR_NOT_CONVERGED : DS_ERROR_NON_FINITE;
     }
     return DS_OK;
+}
+
+ds_status ds_trajectory_jvp(ds_context* c, const ds_state* initial, const ds_control* controls, uint32_t control_count,
+                            const ds_parameters* params, uint32_t steps, const ds_state_tangent* initial_tangent,
+                            const ds_control_tangent* control_tangents, const ds_parameter_tangent* param_tangent, ds_state* final_state,
+                            ds_state_tangent* final_tangent, ds_trajectory_info* info) {
+    if (!c || !initial || !controls || !params || !initial_tangent || !control_tangents || !param_tangent || !final_state ||
+        !final_tangent || (control_count != 1 && control_count != steps)) {
+        set_error("ds_trajectory_jvp: invalid argument (control_count must be 1 or steps)");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    if (c->mode != DS_MODE_DIFFERENTIABLE) {
+        set_error("ds_trajectory_jvp: the context was not created in differentiable mode");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    ACtx x{c, info};
+    if (info) {
+        *info = ds_trajectory_info{};
+        info->steps = steps;
+    }
+    const ds_status st = guard([&] {
+        check_state(initial, "ds_trajectory_jvp: initial state has a null field");
+        check_state(initial_tangent, "ds_trajectory_jvp: initial tangent has a null field");
+        check_state(final_state, "ds_trajectory_jvp: final state has a null field");
+        check_state(final_tangent, "ds_trajectory_jvp: final tangent has a null field");
+        AbiModel m;
+        std::function<const ds_control&(uint32_t)> ctrl = [&](uint32_t s) -> const ds_control& {
+            return controls[control_count == 1 ? 0 : s];
+        };
+        std::function<const ds_control&(uint32_t)> ctrl_t = [&](uint32_t s) -> const ds_control& {
+            return control_tangents[control_count == 1 ? 0 : s];
+        };
+        auto r = xayah::solver::jvp_streaming<AbiModel, ACtx, AState, ds_control, ds_parameters, AState, ds_control, ds_parameters>(
+            m, x, borrow(*initial), ctrl, *params, steps, borrow(*initial_tangent), ctrl_t, *param_tangent);
+        AState out = borrow(*final_state), out_t = borrow(*final_tangent);
+        m.copy_state(r.first, out, x);
+        m.copy_state(r.second, out_t, x);
+        DS_HIP(hipStreamSynchronize(c->stream));
+    });
+    if (st != DS_OK) return st;
+    if (x.soft != DS_OK) set_error("ds_trajectory_jvp: a pressure solve did not converge");
+    return x.soft;
 }
 
 ds_status ds_simulate(ds_context* c, const ds_state* initial, const ds_control* controls, uint32_t control_count,

@@ -87,7 +87,9 @@ struct RhsIn {
     const float* extra[3];    // adjoint only: scalar-advection contributions (may be null)
 };
 
-enum RhsKind : int { kRhsGiven = 0, kRhsForward = 1, kRhsAdjoint = 2 };
+// kRhsTangent: the forward right-hand side of a tangent velocity, without the fixed-face term
+// (the tangent of a fixed pressure value is 0, operators.cu:1015-1018); a host-side kind only.
+enum RhsKind : int { kRhsGiven = 0, kRhsForward = 1, kRhsAdjoint = 2, kRhsTangent = 3 };
 
 // Right-hand side of one cell.
 // forward (pressure_rhs_kernel, operators.cu:875-883): b = -h^2 div(u)/dt + fixed faces;
@@ -321,8 +323,10 @@ void launch_setup(ds_context* c, int kind, const RhsIn& in, const float* x0h, co
                                                                              x0h, x0l, x_is_guess, xh, xl, w.r, w.partials, w.status);
             break;
         case kRhsForward:
-            pcg_setup_kernel<kRhsForward><<<c->pcg.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, c->code, c->mask, in, c->fixed_term,
-                                                                               w.b, x0h, x0l, x_is_guess, xh, xl, w.r, w.partials, w.status);
+        case kRhsTangent:
+            pcg_setup_kernel<kRhsForward><<<c->pcg.blocks, kPcgThreads, 0, c->stream>>>(
+                c->geo, per, c->code, c->mask, in, kind == kRhsTangent ? nullptr : c->fixed_term, w.b, x0h, x0l, x_is_guess, xh, xl, w.r,
+                w.partials, w.status);
             break;
         default:
             pcg_setup_kernel<kRhsAdjoint><<<c->pcg.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, c->code, c->mask, in, c->fixed_term,

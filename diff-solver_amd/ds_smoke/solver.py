@@ -95,6 +95,22 @@ def load_library(path: str | None = None):
         "ds_warm_start_parameter_count": (C.c_uint64, []),
         "ds_warm_start_load": (C.c_int32, [P, P, C.c_uint64, C.c_double]),
         "ds_warm_start_infer": (C.c_int32, [P, P, P]),
+        "ds_jvp_step": (C.c_int32, [P, C.POINTER(abi.State), C.POINTER(abi.Control), C.POINTER(abi.Parameters),
+                                    C.POINTER(abi.State), C.POINTER(abi.StepCache), C.POINTER(abi.State),
+                                    C.POINTER(abi.Control), C.POINTER(abi.Parameters), C.POINTER(abi.State),
+                                    C.POINTER(abi.StepInfo)]),
+        "ds_trajectory_jvp": (C.c_int32, [P, C.POINTER(abi.State), C.POINTER(abi.Control), C.c_uint32,
+                                          C.POINTER(abi.Parameters), C.c_uint32, C.POINTER(abi.State), C.POINTER(abi.Control),
+                                          C.POINTER(abi.Parameters), C.POINTER(abi.State), C.POINTER(abi.State),
+                                          C.POINTER(abi.TrajectoryInfo)]),
+        "ds_buoyancy_jvp": (C.c_int32, [P, P, P, P, P, C.POINTER(P), C.POINTER(abi.Parameters), C.POINTER(abi.Parameters),
+                                        C.POINTER(P)]),
+        "ds_vorticity_jvp": (C.c_int32, [P, C.POINTER(P), C.POINTER(P), C.POINTER(abi.Parameters), C.POINTER(abi.Parameters),
+                                         C.POINTER(P)]),
+        "ds_advect_velocity_jvp": (C.c_int32, [P, C.POINTER(P), C.POINTER(P), C.POINTER(P)]),
+        "ds_constrain_velocity_jvp": (C.c_int32, [P, C.POINTER(P), C.POINTER(P)]),
+        "ds_advect_scalar_jvp": (C.c_int32, [P, C.c_uint32, P, P, C.POINTER(P), C.POINTER(P), P]),
+        "ds_projection_jvp": (C.c_int32, [P, C.POINTER(P), C.POINTER(P), C.POINTER(abi.SolveInfo)]),
         "ds_trajectory_vjp": (C.c_int32, [P, C.POINTER(abi.State), C.POINTER(abi.Control), C.c_uint32,
                                           C.POINTER(abi.Parameters), C.c_uint32, C.c_uint32, C.POINTER(abi.StateAdjoint),
                                           C.POINTER(abi.State), C.POINTER(abi.StateAdjoint), C.POINTER(abi.ControlAdjoint),
@@ -280,6 +296,19 @@ class Context:
             self._check(st, "ds_vjp_step")
         return prev_adj, info.projection
 
+    def jvp_step(self, state, control, params, next_state, cache, state_t, control_t, params_t, next_t=None, check=True):
+        """Model::jvp_step: the tangent of the next state (state / next_state / cache from forward_step)."""
+        next_t = next_t or self.allocate_state()
+        info = abi.StepInfo()
+        st = self.lib.ds_jvp_step(self.h, C.byref(self.state_view(state)), C.byref(self.control_view(control)),
+                                  C.byref(self.params_view(params)), C.byref(self.state_view(next_state)),
+                                  C.byref(self.cache_view(cache)), C.byref(self.state_view(state_t)),
+                                  C.byref(self.control_view(control_t)), C.byref(self.params_view(params_t)),
+                                  C.byref(self.state_view(next_t)), C.byref(info))
+        if check:
+            self._check(st, "ds_jvp_step")
+        return next_t, info.projection
+
     def accumulate_state_adjoint(self, src, dst):
         self._check(self.lib.ds_accumulate_state_adjoint(self.h, C.byref(self.state_view(src)),
                                                          C.byref(self.state_view(dst))), "ds_accumulate_state_adjoint")
@@ -330,6 +359,20 @@ class Context:
             self._check(st, "ds_trajectory_vjp")
         return initial_adj, info
 
+    def trajectory_jvp(self, initial, control, params, steps, initial_t, control_t, params_t, final_state=None, final_t=None,
+                       check=True):
+        """Forward mode over `steps` steps in one sweep: (state_T, its tangent, TrajectoryInfo)."""
+        final_state = final_state or self.allocate_state()
+        final_t = final_t or self.allocate_state()
+        info = abi.TrajectoryInfo()
+        st = self.lib.ds_trajectory_jvp(self.h, C.byref(self.state_view(initial)), C.byref(self.control_view(control)), 1,
+                                        C.byref(self.params_view(params)), steps, C.byref(self.state_view(initial_t)),
+                                        C.byref(self.control_view(control_t)), C.byref(self.params_view(params_t)),
+                                        C.byref(self.state_view(final_state)), C.byref(self.state_view(final_t)), C.byref(info))
+        if check:
+            self._check(st, "ds_trajectory_jvp")
+        return final_state, final_t, info
+
     # ---- operators (tests)
     def masks(self):
         m = np.zeros(cell_shape(self.n), dtype=np.uint8)
@@ -368,6 +411,43 @@ class Context:
         pa = C.byref(self.param_adjoint_view(param_adj)) if param_adj is not None else None
         self._check(self.lib.ds_vorticity_vjp(self.h, _p3(u), C.byref(self.params_view(params)), _p3(fa), _p3(ua), pa),
                     "ds_vorticity_vjp")
+
+    def buoyancy_jvp(self, rho, T, rho_t, T_t, e_t, params, params_t):
+        f = tuple(self.zeros_cell() for _ in range(3))
+        self._check(self.lib.ds_buoyancy_jvp(self.h, _ptr(rho), _ptr(T), _ptr(rho_t), _ptr(T_t), _p3(e_t),
+                                             C.byref(self.params_view(params)), C.byref(self.params_view(params_t)), _p3(f)),
+                    "ds_buoyancy_jvp")
+        return f
+
+    def vorticity_jvp(self, u, u_t, params, params_t, f_t=None):
+        f_t = tuple(self.zeros_cell() for _ in range(3)) if f_t is None else f_t
+        self._check(self.lib.ds_vorticity_jvp(self.h, _p3(u), _p3(u_t), C.byref(self.params_view(params)),
+                                              C.byref(self.params_view(params_t)), _p3(f_t)), "ds_vorticity_jvp")
+        return f_t
+
+    def advect_velocity_jvp(self, u, u_t):
+        out = self.zeros_faces()
+        self._check(self.lib.ds_advect_velocity_jvp(self.h, _p3(u), _p3(u_t), _p3(out)), "ds_advect_velocity_jvp")
+        return out
+
+    def constrain_jvp(self, u_t):
+        out = self.zeros_faces()
+        self._check(self.lib.ds_constrain_velocity_jvp(self.h, _p3(u_t), _p3(out)), "ds_constrain_velocity_jvp")
+        return out
+
+    def advect_scalar_jvp(self, field, src, src_t, u, u_t):
+        out = self.zeros_cell()
+        self._check(self.lib.ds_advect_scalar_jvp(self.h, field, _ptr(src), _ptr(src_t), _p3(u), _p3(u_t), _ptr(out)),
+                    "ds_advect_scalar_jvp")
+        return out
+
+    def projection_jvp(self, u_t, check=True):
+        out = self.zeros_faces()
+        info = abi.SolveInfo()
+        st = self.lib.ds_projection_jvp(self.h, _p3(u_t), _p3(out), C.byref(info))
+        if check:
+            self._check(st, "ds_projection_jvp")
+        return out, info
 
     def integrate_forward(self, u, f):
         out = self.zeros_faces()

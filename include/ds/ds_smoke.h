@@ -160,6 +160,12 @@ typedef struct ds_control_adjoint {
     float* external_acceleration[3];
 } ds_control_adjoint;
 
+/* StateTangent / ControlTangent / ParameterTangent (data.ixx:136-153): FP32 fields with the layout
+ * of their primal twins. */
+typedef ds_state ds_state_tangent;
+typedef ds_control ds_control_tangent;
+typedef ds_parameters ds_parameter_tangent;
+
 typedef struct ds_parameter_adjoint {   /* each a device pointer to ONE double */
     double* ambient_temperature;
     double* density_buoyancy;
@@ -256,6 +262,34 @@ ds_status ds_vorticity_forward(ds_context* c, float* const velocity[3], const ds
 ds_status ds_vorticity_vjp(ds_context* c, float* const velocity[3], const ds_parameters* params,
                            float* const force_adj[3], float* velocity_adj[3],
                            const ds_parameter_adjoint* param_adj);
+/* The tangent operators (the *_jvp launchers, operators.h:40-72). source_jvp and
+ * integrate_velocity_jvp are the forwards on tangents (operators.cu:575-577, 604-606): call
+ * ds_source_forward / ds_integrate_velocity_forward with the tangents. */
+/* buoyancy_jvp (operators.cu:315-324, 589-592). */
+ds_status ds_buoyancy_jvp(ds_context* c, const float* density, const float* temperature,
+                          const float* density_tangent, const float* temperature_tangent,
+                          float* const external_acceleration_tangent[3], const ds_parameters* params,
+                          const ds_parameter_tangent* param_tangent, float* force_tangent[3]);
+/* vorticity_jvp (operators.cu:981-988): force_tangent += the confinement-force tangent.
+ * Differentiable context with confinement on. */
+ds_status ds_vorticity_jvp(ds_context* c, float* const velocity[3], float* const velocity_tangent[3],
+                           const ds_parameters* params, const ds_parameter_tangent* param_tangent,
+                           float* force_tangent[3]);
+/* advect_velocity_jvp (operators.cu:390-413, 618-621): raw self-advection tangent. */
+ds_status ds_advect_velocity_jvp(ds_context* c, float* const velocity[3],
+                                 float* const velocity_tangent[3], float* out_tangent[3]);
+/* constrain_velocity_jvp (operators.cu:633-638): the constraint with zero boundary values and no
+ * collider velocity. */
+ds_status ds_constrain_velocity_jvp(ds_context* c, float* const velocity_tangent[3],
+                                    float* out_tangent[3]);
+/* advect_scalar_jvp (operators.cu:509-535, 650-653); field as in ds_advect_scalar_forward. */
+ds_status ds_advect_scalar_jvp(ds_context* c, uint32_t field, const float* source,
+                               const float* source_tangent, float* const velocity[3],
+                               float* const velocity_tangent[3], float* out_tangent);
+/* projection_jvp (operators.cu:1015-1018) with the PCG solve: the tangent right-hand side has no
+ * fixed-face term. Differentiable context. info may be NULL. */
+ds_status ds_projection_jvp(ds_context* c, float* const velocity_tangent[3], float* out_tangent[3],
+                            ds_solve_info* info);
 /* integrate_velocity_forward (operators.cu:339-364). */
 ds_status ds_integrate_velocity_forward(ds_context* c, float* const velocity[3], float* const force[3],
                                         float* out[3]);
@@ -337,6 +371,18 @@ ds_status ds_vjp_step(ds_context* c, const ds_state* state, const ds_control* co
                       const ds_step_cache* cache, const ds_state_adjoint* next_adj,
                       ds_state_adjoint* prev_adj, ds_control_adjoint* control_adj,
                       const ds_parameter_adjoint* param_adj, ds_step_info* info);
+/* Model::jvp_step (model.cpp:351-359): next_state_tangent = the tangent of the next state for
+ * the tangents of the state, the control and the parameters (param_tangent: four device floats,
+ * as ds_parameters). state, next_state and cache are those of ds_forward_step on the same
+ * inputs (the cache's sourced fields and forced velocity are read; state->velocity with
+ * vorticity confinement on). Needs a differentiable context. The tangent pressure solve starts
+ * from the previous tangent solution when warm_start is DS_WARM_START_PREVIOUS. */
+ds_status ds_jvp_step(ds_context* c, const ds_state* state, const ds_control* control,
+                      const ds_parameters* params, const ds_state* next_state,
+                      const ds_step_cache* cache, const ds_state_tangent* state_tangent,
+                      const ds_control_tangent* control_tangent,
+                      const ds_parameter_tangent* param_tangent,
+                      ds_state_tangent* next_state_tangent, ds_step_info* info);
 
 /* ---- trajectory driver (solver.ixx:53-81,115-150) --------------------------------------- */
 /* Runs `steps` forward steps from `initial` into `final_state`. controls holds control_count
@@ -368,6 +414,18 @@ ds_status ds_trajectory_vjp(ds_context* c, const ds_state* initial, const ds_con
                             ds_state* final_state, ds_state_adjoint* initial_adj,
                             ds_control_adjoint* control_adj, const ds_parameter_adjoint* param_adj,
                             ds_trajectory_info* info);
+
+/* Forward mode over a trajectory, as solver::jvp (solver.ixx:83-113) returns its last element:
+ * one sweep that runs ds_forward_step and ds_jvp_step per step, so no trajectory is stored
+ * (memory: two states, two tangents, one cache). controls and control_tangents each hold
+ * control_count views (1 or `steps`). Writes state_T into final_state and its tangent into
+ * final_tangent. The tangent solves are counted in info's adjoint_* fields. */
+ds_status ds_trajectory_jvp(ds_context* c, const ds_state* initial, const ds_control* controls,
+                            uint32_t control_count, const ds_parameters* params, uint32_t steps,
+                            const ds_state_tangent* initial_tangent,
+                            const ds_control_tangent* control_tangents,
+                            const ds_parameter_tangent* param_tangent, ds_state* final_state,
+                            ds_state_tangent* final_tangent, ds_trajectory_info* info);
 
 #ifdef __cplusplus
 }

@@ -968,7 +968,7 @@ int projection_vjp(Oracle& o, const double* const oa[3], double* const ua[3], ds
 
 // ---- forward mode: the *_jvp kernels and the forwards they reuse ---------------------------
 // The tangent of a fixed boundary value is 0: tangent samples and solves use a copy of the
-// boundary with every value zeroed (operators.cu:396-397, 516-519, 600-605, 1015-1018).
+// boundary with every value zeroed (operators.cu:396-397, 516-519, 633-638, 1015-1018).
 VB zero_values(const VB& b) {
     VB t = b;
     for (double& v : t.values) v = 0.0;
@@ -1062,7 +1062,7 @@ void advect_velocity_jvp(const Oracle& o, const double* const u[3], const double
     }
 }
 
-// constrain_velocity_jvp, operators.cu:599-606: the constraint with zero boundary values and no
+// constrain_velocity_jvp, operators.cu:633-638: the constraint with zero boundary values and no
 // collider velocity.
 void constrain_jvp(const Oracle& o, const double* const u_t[3], double* const out_t[3]) {
     for (int a = 0; a < 3; ++a) {
