@@ -1,7 +1,8 @@
 // xayah/smoke.hpp — C++ API of the MI355X smoke solver, shaped like the reference's
 // xayah.smoke modules (solvers/smoke/data.ixx, model.ixx) and xayah.cuda (solvers/cuda.ixx):
-// Configuration, State/Control/Parameters and their adjoints, Model with allocate_context /
-// allocate_* / copy_* / accumulate_state_adjoint / forward_step / vjp_step.
+// Configuration, State/Control/Parameters with their tangents and adjoints, Model with
+// allocate_context / allocate_* / copy_* / accumulate_state_adjoint / forward_step / jvp_step /
+// vjp_step.
 //
 // Header-only over the C ABI in ds/ds_smoke.h. Differences from the reference, by design:
 //   * adjoint fields are FP32 (adjoint_t), parameter adjoints FP64;
@@ -178,6 +179,11 @@ struct Control {
 struct Parameters {
     hip::Buffer<float> ambient_temperature, density_buoyancy, temperature_buoyancy, vorticity_confinement;
 };
+// StateTangent / ControlTangent / ParameterTangent (data.ixx:136-153): FP32 fields with the
+// members of their primal twins.
+using StateTangent = State;
+using ControlTangent = Control;
+using ParameterTangent = Parameters;
 struct StateAdjoint {
     ScalarAdjointField density;
     ScalarAdjointField temperature;
@@ -399,6 +405,28 @@ struct Model {
         ds_state n = view(next_state);
         ds_step_cache ch = view(cache);
         hip::check(ds_forward_step(c.resource->handle, &s, &k, &p, &n, &ch, &c.last_step), "ds_forward_step");
+    }
+
+    // forward mode (model.ixx:74-76,82)
+    [[nodiscard]] StateTangent allocate_state_tangent(ExecutionContext& c) const { return allocate_state(c); }
+    [[nodiscard]] ControlTangent allocate_control_tangent(ExecutionContext& c) const { return allocate_control(c); }
+    [[nodiscard]] ParameterTangent allocate_parameter_tangent(ExecutionContext& c) const { return allocate_parameters(c); }
+    void copy_state_tangent(const StateTangent& src, StateTangent& dst, ExecutionContext& c) const { copy_state(src, dst, c); }
+
+    // Model::jvp_step (model.cpp:351-359).
+    void jvp_step(const State& state, const Control& control, const Parameters& parameters, const State& next_state,
+                  const StepCache& cache, const StateTangent& state_tangent, const ControlTangent& control_tangent,
+                  const ParameterTangent& parameter_tangent, StateTangent& next_state_tangent, ExecutionContext& c) const {
+        const ds_state s = view(state);
+        const ds_control k = view(control);
+        const ds_parameters p = view(parameters);
+        const ds_state n = view(next_state);
+        const ds_step_cache ch = view(cache);
+        const ds_state_tangent st = view(state_tangent);
+        const ds_control_tangent kt = view(control_tangent);
+        const ds_parameter_tangent pt = view(parameter_tangent);
+        ds_state_tangent nt = view(next_state_tangent);
+        hip::check(ds_jvp_step(c.resource->handle, &s, &k, &p, &n, &ch, &st, &kt, &pt, &nt, &c.last_step), "ds_jvp_step");
     }
 
     // Model::vjp_step (model.cpp:361-389).

@@ -1,12 +1,14 @@
 // facade_check — drives the smoke solver through the C++ facade exactly as code written against
 // the reference's xayah::smoke::Model and xayah::solver would (model.ixx:51-96,
 // solver.ixx:53-150): a closed-box plume trajectory with solver::simulate (store_all), its
-// gradient with solver::vjp, and the same gradient with the checkpointed pair.
+// gradient with solver::vjp, the same gradient with the checkpointed pair, and the tangent of the
+// final state with solver::jvp over the stored trajectory and with the one-sweep jvp_streaming.
 //
 // usage: facade_check <dir> <n> <steps> <k>
 //   reads  <dir>/u0_{x,y,z}.f32 (initial face velocity), source.f32 (density and temperature
-//          source), target.f32 (target density)
-//   writes <dir>/final_density.f32, grad_store_{x,y,z}.f32, grad_ckpt_{x,y,z}.f32, loss.txt
+//          source), target.f32 (target density), v_{x,y,z}.f32 (initial velocity tangent)
+//   writes <dir>/final_density.f32, grad_store_{x,y,z}.f32, grad_ckpt_{x,y,z}.f32, loss.txt,
+//          tangent_store.f32, tangent_stream.f32 (the final density tangent of each driver)
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -117,6 +119,24 @@ int main(int argc, char** argv) {
         solver::vjp_checkpointed(model, ctx, t, ctl, par, final_adj, ctl_adj_of, par_adj, init_adj);
         const hip::Buffer<float>* gc[3] = {&init_adj.velocity.x, &init_adj.velocity.y, &init_adj.velocity.z};
         for (int a = 0; a < 3; ++a) write_f32(dir + "/grad_ckpt_" + axis[a] + ".f32", download(ctx, *gc[a]));
+        // forward mode: solver::jvp over the stored trajectory, and the one-sweep driver
+        smoke::StateTangent v0 = model.allocate_state_tangent(ctx);
+        hip::Buffer<float>* vt[3] = {&v0.velocity.x, &v0.velocity.y, &v0.velocity.z};
+        for (int a = 0; a < 3; ++a) {
+            const std::vector<float> v = read_f32(dir + "/v_" + axis[a] + ".f32", vt[a]->size);
+            ctx.resource->copy_from_host(vt[a]->data, v.data(), v.size() * sizeof(float));
+        }
+        std::vector<smoke::ControlTangent> ctl_t;
+        for (std::uint32_t s = 0; s < steps; ++s) ctl_t.push_back(model.allocate_control_tangent(ctx));
+        const smoke::ParameterTangent par_t = model.allocate_parameter_tangent(ctx);
+        auto tangents = solver::jvp(model, ctx, traj, std::span<const smoke::Control>(controls), par, v0,
+                                    std::span<const smoke::ControlTangent>(ctl_t), par_t);
+        write_f32(dir + "/tangent_store.f32", download(ctx, tangents.back().density.values));
+        std::function<const smoke::ControlTangent&(std::uint32_t)> ctl_t_of = [&](std::uint32_t s) -> const smoke::ControlTangent& {
+            return ctl_t[s];
+        };
+        auto streamed = solver::jvp_streaming(model, ctx, s0, ctl, par, steps, v0, ctl_t_of, par_t);
+        write_f32(dir + "/tangent_stream.f32", download(ctx, streamed.second.density.values));
         std::ofstream(dir + "/loss.txt") << std::to_string(loss) << "\n";
         std::printf("facade_check: n=%u steps=%u k=%u loss=%.9g\n", n, steps, k, loss);
     } catch (const std::exception& e) {
