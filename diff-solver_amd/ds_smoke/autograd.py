@@ -3,7 +3,9 @@
 forward  = ds_simulate (the forward steps, model.cpp:341-349, driven as solver::simulate,
            solver.ixx:53-81);
 backward = ds_trajectory_vjp (the hand-written VJP, model.cpp:361-389, driven as solver::vjp,
-           solver.ixx:115-150, with a checkpoint every `checkpoint_interval` steps and recompute).
+           solver.ixx:115-150, with a checkpoint every `checkpoint_interval` steps and recompute);
+jvp      = ds_trajectory_jvp (forward mode, model.cpp:351-359, as solver::jvp, solver.ixx:83-113,
+           in one sweep), for torch.autograd.forward_ad / torch.func.jvp.
 
 Tensors stay on the GPU (zero copy): the C ABI reads and writes them in place. The control is the
 same at every step, so its gradient is the sum over steps. Parameters are one float32 tensor
@@ -69,6 +71,28 @@ class SmokeRollout(torch.autograd.Function):
         fctx.steps = steps
         fctx.checkpoint_interval = checkpoint_interval
         fctx.save_for_backward(params, *state, *control)
+        fctx.save_for_forward(params, *state, *control)
+        return tuple(outs)
+
+    @staticmethod
+    def jvp(fctx, _solver, _steps, _checkpoint_interval, params_t, *fields_t):
+        """Forward mode: ds_trajectory_jvp (solver::jvp, solver.ixx:83-113, in one sweep that runs the
+        forward again); a missing tangent is zero."""
+        solver = fctx.solver
+        saved = fctx.saved_tensors
+        params, state, control = saved[0], list(saved[1:6]), list(saved[6:11])
+        zero_or = lambda t, like: t.detach().contiguous() if t is not None else torch.zeros_like(like)
+        state_t = [zero_or(t, s) for t, s in zip(fields_t[:5], state)]
+        control_t = [zero_or(t, k) for t, k in zip(fields_t[5:], control)]
+        params_t = zero_or(params_t, params)
+        current = torch.cuda.current_stream(solver.device)
+        solver.stream.wait_stream(current)
+        _, final_t, _ = solver.trajectory_jvp(_state(state), _control(control), params, fctx.steps, _state(state_t),
+                                              _control(control_t), params_t)
+        current.wait_stream(solver.stream)
+        outs = _flat_state(final_t)
+        for t in outs:
+            t.record_stream(current)
         return tuple(outs)
 
     @staticmethod

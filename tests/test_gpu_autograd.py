@@ -118,3 +118,36 @@ def test_density_loss_through_autograd_equals_loss_gradient():
     ref = ctx.to_host(g_ref)["velocity"]
     for a in range(3):
         assert rel(state["velocity"][a].grad.cpu().numpy(), ref[a]) < 1e-5
+
+
+def test_forward_mode_through_autograd():
+    """torch.autograd.forward_ad on the rollout: the tangent equals ds_trajectory_jvp's, and its
+    directional derivative of the density loss equals the reverse sweep's."""
+    import torch
+    import torch.autograd.forward_ad as fwAD
+    from ds_smoke import abi, autograd, scenes
+    from ds_smoke.solver import Context
+
+    sc = scenes.plume(16)
+    ctx = Context(sc.config(tolerance=1e-6), abi.MODE_DIFFERENTIABLE)
+    s0, ctl = ctx.state_from_host(scenes.initial_state(sc)), ctx.control_from_host(scenes.control(sc))
+    params = ctx.allocate_parameters(sc.params)
+    rng = np.random.default_rng(9)
+    v = tuple(ctx.to_device(rng.standard_normal(tuple(x.shape)).astype(np.float32)) for x in s0["velocity"])
+    steps = 6
+    with fwAD.dual_level():
+        dual = dict(s0, velocity=tuple(fwAD.make_dual(u, t) for u, t in zip(s0["velocity"], v)))
+        out = autograd.rollout(ctx, params, dual, ctl, steps)
+        primal, tangent = fwAD.unpack_dual(out["density"])
+    torch.cuda.synchronize()
+    zeros = lambda d: {k: (tuple(torch.zeros_like(x) for x in val) if isinstance(val, tuple) else torch.zeros_like(val)) for k, val in d.items()}
+    st_t = dict(zeros(s0), velocity=v)
+    fin, fin_t, _ = ctx.trajectory_jvp(s0, ctl, params, steps, st_t, zeros(ctl), torch.zeros_like(params))
+    np.testing.assert_array_equal(tangent.cpu().numpy(), ctx.to_host(fin_t)["density"])
+    np.testing.assert_array_equal(primal.cpu().numpy(), ctx.to_host(fin)["density"])
+    target = ctx.to_device(sc.target)
+    _, grad, _ = ctx.loss_gradient(s0, ctl, params, steps, 3, target)
+    fwd = float(((primal.double() - target.double()) * tangent.double()).sum())
+    rev = sum(float((g.double() * t.double()).sum()) for g, t in zip(grad["velocity"], v))
+    print(f"forward mode {fwd:.8e}, reverse mode {rev:.8e}")
+    assert abs(fwd - rev) < 2e-3 * abs(rev)
