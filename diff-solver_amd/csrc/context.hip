@@ -93,8 +93,7 @@ void validate(const ds_config& cfg, uint32_t mode) {
         for (int a = 0; a < 3; ++a)
             if (!(col.size[a] > 0.0F)) bad("collider size must be positive");
     }
-    if (cfg.projection_solver != DS_PROJECTION_PCG)
-        throw Failure{DS_ERROR_UNSUPPORTED, "only the PCG projection solver runs on the GPU"};
+    if (cfg.projection_solver > DS_PROJECTION_RBGS) bad("projection_solver out of range");
     if (cfg.warm_start > DS_WARM_START_CNN) bad("warm_start out of range");
     if (cfg.warm_start == DS_WARM_START_CNN && (cfg.resolution[0] % 64 || cfg.resolution[1] % 16 || cfg.resolution[2] % 4))
         throw Failure{DS_ERROR_UNSUPPORTED, "the warm-start network needs nx % 64 == 0, ny % 16 == 0 and nz % 4 == 0"};
@@ -343,11 +342,17 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
             c->vb_zero = c->vb;
             c->db_zero = c->db;
             c->tb_zero = c->tb;
+            c->pb_zero = c->pb;
             for (float& v : c->vb_zero.values) v = 0.0f;
             for (float& v : c->db_zero.values) v = 0.0f;
             for (float& v : c->tb_zero.values) v = 0.0f;
+            for (float& v : c->pb_zero.values) v = 0.0f;
             VBound* v = device_alloc<VBound>(c, 2);
-            SBound* sb = device_alloc<SBound>(c, 4);
+            SBound* sb = device_alloc<SBound>(c, 6);
+            DS_HIP(hipMemcpyAsync(sb + 4, &c->pb, sizeof(SBound), hipMemcpyHostToDevice, c->stream));
+            DS_HIP(hipMemcpyAsync(sb + 5, &c->pb_zero, sizeof(SBound), hipMemcpyHostToDevice, c->stream));
+            c->d_pb = sb + 4;
+            c->d_pb_zero = sb + 5;
             DS_HIP(hipMemcpyAsync(v, &c->vb, sizeof(VBound), hipMemcpyHostToDevice, c->stream));
             DS_HIP(hipMemcpyAsync(v + 1, &c->vb_zero, sizeof(VBound), hipMemcpyHostToDevice, c->stream));
             DS_HIP(hipMemcpyAsync(sb, &c->db, sizeof(SBound), hipMemcpyHostToDevice, c->stream));
@@ -452,6 +457,10 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
             }
             c->tangent_hi = device_alloc<float>(c, n);
             c->tangent_lo = device_alloc<float>(c, n);
+        }
+        if (cfg.projection_solver == DS_PROJECTION_RBGS) {
+            c->rbgs_diag = device_alloc<uint8_t>(c, n);
+            rbgs_prepare(c);
         }
         if (cfg.vorticity_confinement_enabled) {
             c->vorticity = true;

@@ -209,6 +209,11 @@ struct ds_context {
     const ds::VBound* d_vb_zero = nullptr;
     const ds::SBound* d_db_zero = nullptr;
     const ds::SBound* d_tb_zero = nullptr;
+    // pressure boundary (device) for the red-black Gauss-Seidel mode, and its zeroed copy
+    const ds::SBound* d_pb = nullptr;
+    const ds::SBound* d_pb_zero = nullptr;
+    ds::SBound pb_zero{};
+    uint8_t* rbgs_diag = nullptr;     // RBGS diagonal per cell, 0 = not updated (DS_PROJECTION_RBGS)
 
     // vorticity confinement (device; allocated when vorticity_confinement_enabled)
     bool vorticity = false;
@@ -249,6 +254,10 @@ ds_solve_info pcg_solve(ds_context* c, float* x_hi, float* x_lo, const float* x0
 ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], const float* const extra[3], float* xh,
                              float* xl, const float* x0h, const float* x0l, double tol, uint32_t max_it);
 uint32_t pcg_periodic_bits(const ds_context* c);
+// DS_PROJECTION_RBGS (rbgs.hip): the diagonal table, then the solve of a right-hand-side kind
+// (1 forward, 2 the unrolled reverse, 3 tangent) into xh with xl = 0.
+void rbgs_prepare(ds_context* c);
+ds_solve_info rbgs_solve_kind(ds_context* c, int kind, const float* const u[3], const float* const extra[3], float* xh, float* xl);
 // out = hi + lo (rounded to FP32).
 void combine_pair(ds_context* c, const float* hi, const float* lo, float* out);
 

@@ -28,6 +28,7 @@
 #include "context.hpp"
 #include "pcg_fused.hpp"
 #include "pcg_plane.hpp"
+#include "rhs.hpp"
 
 namespace ds {
 
@@ -80,47 +81,6 @@ __device__ __forceinline__ T neighbour_sum(const Geo& g, uint32_t per, int x, in
 #pragma unroll
     for (int k = 0; k < 6; ++k) sum += (T)s.w[k] * v(s.j[k]);
     return sum;
-}
-
-struct RhsIn {
-    const float* u[3];        // forward: advected velocity; adjoint: projected-velocity adjoint
-    const float* extra[3];    // adjoint only: scalar-advection contributions (may be null)
-};
-
-// kRhsTangent: the forward right-hand side of a tangent velocity, without the fixed-face term
-// (the tangent of a fixed pressure value is 0, operators.cu:1015-1018); a host-side kind only.
-enum RhsKind : int { kRhsGiven = 0, kRhsForward = 1, kRhsAdjoint = 2, kRhsTangent = 3 };
-
-// Right-hand side of one cell.
-// forward (pressure_rhs_kernel, operators.cu:875-883): b = -h^2 div(u)/dt + fixed faces;
-// adjoint (project_velocity_reverse_kernel, operators.cu:922-933, as a gather): p_bar_i =
-//   (dt/h) * sum_a [u_bar(i+e_a) - u_bar(i)] over projectable faces (both cells fluid, in domain).
-template <int Kind>
-__device__ __forceinline__ float cell_rhs(const Geo& g, const RhsIn& in, const uint8_t* __restrict__ mask,
-                                          const float* __restrict__ fixed_term, uint32_t i, int x, int y, int z) {
-    const uint32_t ex0 = (uint32_t)g.nx + 1u, ey1 = (uint32_t)g.ny + 1u;
-    const uint32_t f0 = (uint32_t)x + ex0 * ((uint32_t)y + (uint32_t)g.ny * (uint32_t)z);
-    const uint32_t f1 = (uint32_t)x + (uint32_t)g.nx * ((uint32_t)y + ey1 * (uint32_t)z);
-    const uint32_t f2 = i;
-    const uint32_t sz = (uint32_t)g.nx * (uint32_t)g.ny;
-    if (Kind == kRhsForward) {
-        const float sum = (in.u[0][f0 + 1u] - in.u[0][f0]) + (in.u[1][f1 + (uint32_t)g.nx] - in.u[1][f1]) +
-                          (in.u[2][f2 + sz] - in.u[2][f2]);
-        const float divergence = sum * g.inv_h;
-        float b = -g.h * g.h * divergence / g.dt;
-        if (fixed_term) b += fixed_term[i];
-        return b;
-    } else {
-        auto ua = [&](int a, uint32_t f) { return in.u[a][f] + (in.extra[a] ? in.extra[a][f] : 0.0f); };
-        float s = 0.0f;
-        if (x + 1 < g.nx && mask[i + 1u] == 0) s += ua(0, f0 + 1u);
-        if (x > 0 && mask[i - 1u] == 0) s -= ua(0, f0);
-        if (y + 1 < g.ny && mask[i + (uint32_t)g.nx] == 0) s += ua(1, f1 + (uint32_t)g.nx);
-        if (y > 0 && mask[i - (uint32_t)g.nx] == 0) s -= ua(1, f1);
-        if (z + 1 < g.nz && mask[i + sz] == 0) s += ua(2, f2 + sz);
-        if (z > 0 && mask[i - sz] == 0) s -= ua(2, f2);
-        return g.dt * s * g.inv_h;
-    }
 }
 
 // Setup: b (computed or given), x = x0 (or 0), r = b - A x0 (FP64), partials ||b||^2, ||r||^2, <r,z>.
@@ -570,6 +530,7 @@ uint32_t pcg_periodic_bits(const ds_context* c) { return periodic_bits(c); }
 // Solves A x = b for the given right-hand-side kind. Returns the solve info; x = (xh, xl).
 ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], const float* const extra[3], float* xh,
                              float* xl, const float* x0h, const float* x0l, double tol, uint32_t max_it) {
+    if (c->config.projection_solver == DS_PROJECTION_RBGS && kind != kRhsGiven) return rbgs_solve_kind(c, kind, u, extra, xh, xl);
     RhsIn in{};
     for (int a = 0; a < 3; ++a) {
         in.u[a] = u ? u[a] : nullptr;

@@ -169,3 +169,39 @@ def test_trajectory_jvp_gives_the_reverse_sweeps_directional_derivative(vorticit
     forward_mode = _dot(fin["density"] - sc.target, fin_t["density"])
     reverse_mode = _dot(ref["grad"]["velocity"], v["velocity"]) + float(np.dot(ref["params"], pt))
     assert abs(forward_mode - reverse_mode) < 1e-8 * abs(reverse_mode), (forward_mode, reverse_mode)
+
+
+@pytest.mark.parametrize("name", ["closed", "solids", "periodic"])
+def test_rbgs_mode_projection_reverse_and_tangent(name):
+    """The reference's fixed-cost projection (red-black Gauss-Seidel from p = 0, 7 sweeps of each
+    colour): its tangent equals projection(u + v) - projection(u) (an affine map), its reverse
+    is the transpose of that map, and a full step's JVP and VJP are each other's transpose."""
+    from ds_smoke import abi
+
+    sc = _scene(name)
+    o = O.Oracle(sc, solver=abi.PROJECTION_RBGS, rbgs_iterations=7)
+    rng = np.random.default_rng(35)
+    n = sc.resolution
+    u = tuple(rng.standard_normal(face_shape(n, a)) for a in range(3))
+    du = tuple(rng.standard_normal(face_shape(n, a)) for a in range(3))
+    w = tuple(rng.standard_normal(face_shape(n, a)) for a in range(3))
+    pj, _ = o.projection_jvp(du)
+    base = o.projection_forward(u)[0]
+    pd = tuple(p - b for p, b in zip(o.projection_forward(tuple(x + y for x, y in zip(u, du)))[0], base))
+    for a in range(3):
+        assert rel(pj[a], pd[a]) < 1e-10, (name, a, rel(pj[a], pd[a]))
+    pa, _ = o.projection_vjp(w)
+    assert abs(_dot(pj, w) - _dot(du, pa)) < 1e-10 * abs(_dot(pj, w))
+    state, control = _random_state(sc, rng), _random_control(sc, rng)
+    params = np.array(sc.params, dtype=np.float64)
+    nxt, cache, _ = o.forward_step(state, control, params)
+    st_t, ct_t = _random_state(sc, rng, 1.0), _random_control(sc, rng, 1.0)
+    pt = np.array([0.3, -0.2, 0.5, 0.0])
+    jv, _ = o.jvp_step(state, nxt, cache, params, st_t, ct_t, pt)
+    ws = _random_state(sc, rng, 1.0)
+    ca = {"density_source": o.zeros_cell(), "temperature_source": o.zeros_cell(),
+          "external_acceleration": tuple(o.zeros_cell() for _ in range(3))}
+    qa = np.zeros(4)
+    prev, _ = o.vjp_step(nxt, cache, params, ws, control_adj=ca, param_adj=qa, state=state)
+    lhs, rhs = _dot(jv, ws), _dot(st_t, prev) + _dot(ct_t, ca) + float(np.dot(pt, qa))
+    assert abs(lhs - rhs) < 1e-9 * (abs(lhs) + 1.0), (lhs, rhs)
