@@ -207,7 +207,8 @@ def plume(n: int, noise_seed: int = 1, target_seed: int = 2, solids: int = 0, so
     if solids:
         # radius U[8, 20] cells at 256^3, scaled with the grid below that
         scale = min(1.0, n / 256.0)
-        sc.colliders = random_spheres(res, solid_seed, solids, (8.0 * scale, 20.0 * scale), source_centre,
+        # placed against the canonical source, so plumes with a jittered source share the spheres
+        sc.colliders = random_spheres(res, solid_seed, solids, (8.0 * scale, 20.0 * scale), SOURCE_CENTRE,
                                       source_radius, float(h))
     sc.name = f"plume{n}" + (f"_solids{solids}" if solids else "")
     return sc

@@ -20,15 +20,15 @@ import time
 import numpy as np
 
 
-def evaluate(weights, output_scale, grid, runs, steps, seed, device=0, log=print):
+def evaluate(weights, output_scale, grid, runs, steps, seed, device=0, log=print, solids=0):
     import torch
 
     from . import abi, scenes
     from .solver import Context
     from .warm_start import generate_pairs
 
-    bs, ps, unknown, _ = generate_pairs(grid, runs, steps, seed=seed, tolerance=1e-8, device=device, log=log)
-    sc = scenes.plume(grid, with_target=False)
+    bs, ps, unknown, _ = generate_pairs(grid, runs, steps, seed=seed, tolerance=1e-8, device=device, log=log, solids=solids)
+    sc = scenes.plume(grid, with_target=False, solids=solids)
     plain = Context(sc.config(tolerance=1e-6), abi.MODE_FORWARD, device)
     cnn = Context(sc.config(tolerance=1e-6, warm_start=abi.WARM_START_CNN), abi.MODE_FORWARD, device)
     cnn.warm_start_load(weights, output_scale)
@@ -79,6 +79,7 @@ def main(argv=None):
     ap.add_argument("--eval-runs", type=int, default=2)
     ap.add_argument("--fp32", action="store_true", help="train without BF16 autocast")
     ap.add_argument("--init", default=None, help="start from these parameters (a file written by this script)")
+    ap.add_argument("--solids", type=int, default=0, help="seeded solid spheres in every run (the same placement)")
     ap.add_argument("--out", required=True)
     args = ap.parse_args(argv)
 
@@ -86,18 +87,19 @@ def main(argv=None):
 
     log = lambda m: print(m, file=sys.stderr, flush=True)
     t0 = time.time()
-    bs, ps, unknown, diag = warm_start.generate_pairs(args.grid, args.runs, args.steps, seed=100, log=log)
+    bs, ps, unknown, diag = warm_start.generate_pairs(args.grid, args.runs, args.steps, seed=100, log=log, solids=args.solids)
     print(json.dumps({"phase": "data", "pairs": len(bs), "seconds": round(time.time() - t0, 1)}), flush=True)
     init = warm_start.load(args.init)[0] if args.init else None
     model, output_scale = warm_start.train(bs, ps, unknown, epochs=1000, batch=args.batch, lr=args.lr,
                                            time_budget=60.0 * args.minutes, log=log, diag=diag, amp=not args.fp32, init=init)
     flat = warm_start.flatten_parameters(model)
-    meta = {"grid": args.grid, "eval_grid": args.eval_grid, "runs": args.runs, "steps": args.steps, "pairs": len(bs), "minutes": args.minutes,
+    meta = {"grid": args.grid, "eval_grid": args.eval_grid, "solids": args.solids, "runs": args.runs, "steps": args.steps, "pairs": len(bs),
+            "minutes": args.minutes,
             "batch": args.batch, "lr": args.lr, "label_tolerance": 1e-8}
     warm_start.save(args.out, flat, output_scale, meta)
     print(json.dumps({"phase": "train", "output_scale": output_scale, "seconds": round(time.time() - t0, 1)}), flush=True)
     del bs, ps
-    ev = evaluate(flat, output_scale, args.eval_grid, args.eval_runs, min(args.steps, 32), seed=900, log=log)
+    ev = evaluate(flat, output_scale, args.eval_grid, args.eval_runs, min(args.steps, 32), seed=900, log=log, solids=args.solids)
     print(json.dumps({"phase": "eval", **ev}), flush=True)
 
 

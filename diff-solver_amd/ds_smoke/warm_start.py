@@ -92,8 +92,9 @@ def load(path: str):
 
 
 def generate_pairs(n: int, runs: int, steps: int, seed: int = 100, jitter_cells: float = 8.0, tolerance: float = 1e-8,
-                   device: int = 0, log=print):
-    """(b, p) pairs from forward steps of jittered plumes at n^3, kept on the GPU."""
+                   device: int = 0, log=print, solids: int = 0, solid_seed: int = 4):
+    """(b, p) pairs from forward steps of jittered plumes at n^3, kept on the GPU. With solids, every
+    run has the same `solids` seeded spheres (seed solid_seed), so the unknowns are shared."""
     import torch
 
     from .solver import Context
@@ -104,7 +105,7 @@ def generate_pairs(n: int, runs: int, steps: int, seed: int = 100, jitter_cells:
     for r in range(runs):
         off = rng.uniform(-jitter_cells, jitter_cells, 3) / n
         centre = (0.5 + off[0], 0.125 + abs(off[1]) * 0.5, 0.5 + off[2])
-        sc = scenes.plume(n, noise_seed=seed + 1 + r, source_centre=centre, with_target=False)
+        sc = scenes.plume(n, noise_seed=seed + 1 + r, source_centre=centre, with_target=False, solids=solids, solid_seed=solid_seed)
         ctx = Context(sc.config(tolerance=tolerance, max_iterations=6000), abi.MODE_FORWARD, device)
         state = ctx.state_from_host(scenes.initial_state(sc))
         ctl = ctx.control_from_host(scenes.control(sc))
