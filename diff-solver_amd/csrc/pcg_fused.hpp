@@ -30,17 +30,23 @@
 // relative to those steps, not to x, and the pair keeps the accuracy it needs (FP32 x alone stalls
 // above 1e-6); each iteration touches 4 bytes of solution instead of the pair's 8.
 //
-// Layout: the plane march of the two-kernel path — a 1024-thread block owns a 64 x 16 (x, y)
+// Layout: the plane march of the two-kernel path — a block owns a 64 x (threads / 64) (x, y)
 // tile and a chunk of z planes; u of the plane being differentiated sits in LDS (own cell plus
 // the tile rim), its z neighbours in registers. The raw loads of plane z+1 are consumed entirely
-// while plane z is computed and those of plane z+3 are issued then; the z loop is unrolled by two
-// over two named load slots, so a slot is never copied and the loads stay in flight for two
-// plane steps.
+// while plane z is computed and those of plane z+1+kFusedDepth are issued then; the z loop is
+// unrolled over kFusedDepth load slots (compile-time indices, so a slot is never copied) and the
+// loads stay in flight for kFusedDepth plane steps.
 #pragma once
 
 #include "pcg_plane.hpp"
 
 namespace ds {
+
+// Planes whose loads are in flight ahead of the plane being computed. Measured on the 128^3
+// plume: depth 2, 28.3 us per launch; depth 4 (122 VGPRs, still two blocks per CU), 28.7 us; at
+// 256^3 both stream about 4.2 TB/s, so the 128^3 launch is bound by its fixed cost (launch,
+// partial reduction, tail), not by the prefetch depth.
+constexpr int kFusedDepth = 2;
 
 // Kernel modes: the first iteration after set-up (no previous direction), a general iteration,
 // and the priming pass w = A u, partials gamma, delta, rho (after set-up or a residual replacement).
@@ -115,7 +121,11 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fused_kernel(Geo g, uint32_t 
                                                          uint32_t check_only, uint32_t respect_stop, uint32_t with_ps) {
     __shared__ float tile[2][kTY + 2][kTX + 2];
     __shared__ double lds[7 * kMaxWaves];
-    if ((Mode != kFusedPrime || respect_stop) && status->stop) return;
+    // the stop flag, gamma_{i-1} and the producer's partials (below) are independent loads issued
+    // together; the plane loads wait for the stop flag (launches queued after convergence must
+    // not stream the vectors), gamma_{i-1} is not waited for until beta is formed
+    const uint32_t stop = (Mode != kFusedPrime || respect_stop) ? status->stop : 0u;
+    const double gamma_prev = Mode == kFusedNext ? hist[2 * (k - 1)] : 1.0;
     const uint32_t lb = xcd_block();
     int x0, y0, z0, z1;
     plane_block(g, pt, lb, x0, y0, z0, z1);
@@ -140,6 +150,8 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fused_kernel(Geo g, uint32_t 
         for (int q = 0; q < 6; ++q) part[q] = in_parts[kSlots[q] * kPcgMaxBlocks + tp];
         part[6] = bbp[kPartBB * kPcgMaxBlocks + tp];
     }
+
+    if (stop) return;   // block-uniform, before any barrier
 
     // loads of plane zz: the own column up to the top halo plane z1 (planes beyond read the pad);
     // rims for the chunk planes; the own-cell update values for every plane (used only where the
@@ -166,12 +178,12 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fused_kernel(Geo g, uint32_t 
     };
 
     FHalo below_raw{0.0f, 0.0f, 0.0f, 0u};
-    FPlane cur, slot_a, slot_b;
+    FPlane cur, slots[kFusedDepth];
     if (z0 < z1) {
         below_raw = fused_load_halo<Mode>(halo_index(g, wrapped_index(g, per, x, y, z0 - 1)), v, code);
         load_plane(z0, cur);
-        load_plane(z0 + 1, slot_a);
-        load_plane(z0 + 2, slot_b);
+#pragma unroll
+        for (int j = 0; j < kFusedDepth; ++j) load_plane(z0 + 1 + j, slots[j]);
     }
 
     float alpha = 0.0f, beta = 0.0f;
@@ -193,7 +205,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fused_kernel(Geo g, uint32_t 
         }
         double b64 = 0.0, a64;
         if (Mode == kFusedNext) {
-            b64 = gamma / hist[2 * (k - 1)];
+            b64 = gamma / gamma_prev;
             a64 = gamma / (delta + b64 * (red[3] + red[4]) + b64 * b64 * red[5]);
         } else {
             a64 = gamma / delta;
@@ -252,8 +264,9 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fused_kernel(Geo g, uint32_t 
         __syncthreads();
         int buf = 0;
         // one plane step: consume the slot holding plane z+1, finish plane z, refill the slot with
-        // z+3. A step with z >= z1 (the second step of an odd-length chunk) only loads and waits,
-        // so that both steps of the unrolled loop run unconditionally.
+        // z+1+kFusedDepth. A step with z >= z1 (past the end of a chunk whose length is not a
+        // multiple of the depth) only loads and waits, so every step of the unrolled loop runs
+        // unconditionally.
         auto step = [&](FPlane& S, int z) {
             float pn = 0.0f;
             const float next = own_update(S, z + 1, pn);
@@ -273,7 +286,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fused_kernel(Geo g, uint32_t 
                 wp += (double)wv * pc;
             }
             const uint8_t dn = S.c.d;
-            load_plane(z + 3, S);
+            load_plane(z + 1 + kFusedDepth, S);
             __syncthreads();
             below = centre;
             centre = next;
@@ -281,9 +294,9 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fused_kernel(Geo g, uint32_t 
             d = dn;
             buf ^= 1;
         };
-        for (int z = z0; z < z1; z += 2) {
-            step(slot_a, z);
-            step(slot_b, z + 1);
+        for (int z = z0; z < z1; z += kFusedDepth) {
+#pragma unroll
+            for (int j = 0; j < kFusedDepth; ++j) step(slots[j], z + j);
         }
     }
     double red[6]{rho, gam, del, wp, su, sp};
