@@ -351,6 +351,7 @@ struct Model {
     static void set_pressure_out(StepCache& cache, float* p) { cache.pressure = p; }
     static void set_pressure_guess(StepCache& cache, const float* p) { cache.pressure_guess = p; }
     void zero_state_adjoint(StateAdjoint& a, ExecutionContext& c) const { zero(c, a); }
+    void zero_state(State& s, ExecutionContext& c) const { zero(c, s); }
     static void after_forward(ExecutionContext&, int) {}
     static void after_vjp(ExecutionContext&) {}
     [[nodiscard]] StateAdjoint allocate_state_adjoint(ExecutionContext& c) const {

@@ -434,6 +434,7 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
         c->pcg.partials = device_alloc<double>(c, 2 * (size_t)kPartCount * kPcgMaxBlocks);   // two sets
         c->pcg.hist = device_alloc<double>(c, 2 * ((size_t)c->max_iterations + 4));
         c->pcg.status = device_alloc<PcgStatus>(c, 1);
+        c->metric_partials = device_alloc<double>(c, 6 * (size_t)kMetricBlocks);
         void* hs = nullptr;
         DS_HIP(hipHostMalloc(&hs, 2 * sizeof(PcgStatus), hipHostMallocDefault));
         c->pcg.host_status = static_cast<PcgStatus*>(hs);

@@ -64,6 +64,8 @@ inline uint32_t xcd_blocks_for(uint64_t count, uint32_t block = kBlock) { return
 // Reduction layout: kernels of kPcgBlocks (256-thread) blocks leave one FP64 partial per block
 // per quantity; consumers finish the sum in a fixed order (deterministic, no atomics).
 constexpr uint32_t kPcgBlocks = 512;
+// Blocks of the example metric reduction (one partial per block per metric, finished in order).
+constexpr uint32_t kMetricBlocks = 512;
 // PCG iteration kernels: 1024-thread blocks, each over a contiguous range of whole multiples of
 // kPcgUnroll * 1024 cells; the block count (<= kPcgMaxBlocks, a multiple of 8) is per context.
 // (measured on the 128^3 plume: 512 threads / ~512 blocks, i.e. two independent 8-wave blocks per
@@ -194,6 +196,7 @@ struct ds_context {
     float* lambda_lo = nullptr;
     bool lambda_valid = false;
     double* param_partials = nullptr; // 4 * kPcgBlocks
+    double* metric_partials = nullptr; // 6 * kMetricBlocks (ds_example_reduce_metrics)
 
     // forward-mode (tangent) scratch (device, differentiable mode)
     float* sourced_t[2]{};            // tangents of rho_s, T_s

@@ -114,6 +114,19 @@ class TrajectoryInfo(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class JetControl(C.Structure):
+    """ds_jet_control: the double-jet control of the forward example (simulation.h:10)."""
+    _fields_ = [("pulse_period", C.c_float), ("left_source_center", C.c_float * 3), ("right_source_center", C.c_float * 3),
+                ("source_radius", C.c_float), ("density_source_rate", C.c_float), ("temperature_source_rate", C.c_float),
+                ("left_acceleration", C.c_float * 3), ("right_acceleration", C.c_float * 3)]
+
+    def values(self):
+        """The 16 values in field order."""
+        return ([self.pulse_period] + list(self.left_source_center) + list(self.right_source_center)
+                + [self.source_radius, self.density_source_rate, self.temperature_source_rate]
+                + list(self.left_acceleration) + list(self.right_acceleration))
+
+
 class KernelTiming(C.Structure):
     _fields_ = [("name", C.c_char * 48), ("launches", C.c_uint64), ("samples", C.c_uint64), ("total_ms", C.c_double)]
 

@@ -115,6 +115,8 @@ def load_library(path: str | None = None):
                                           C.POINTER(abi.Parameters), C.c_uint32, C.c_uint32, C.POINTER(abi.StateAdjoint),
                                           C.POINTER(abi.State), C.POINTER(abi.StateAdjoint), C.POINTER(abi.ControlAdjoint),
                                           C.POINTER(abi.ParameterAdjoint), C.POINTER(abi.TrajectoryInfo)]),
+        "ds_example_write_control": (C.c_int32, [P, C.c_uint64, C.POINTER(abi.JetControl), P, P, C.POINTER(P)]),
+        "ds_example_reduce_metrics": (C.c_int32, [P, P, P, C.POINTER(P), C.POINTER(P), P]),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -269,6 +271,22 @@ class Context:
         v.pressure = ch["pressure"].data_ptr() if ch.get("pressure") is not None else None
         v.pressure_guess = ch["pressure_guess"].data_ptr() if ch.get("pressure_guess") is not None else None
         return v
+
+    # ---- forward example harness (examples/smoke/forward/simulation.cu)
+    def example_write_control(self, step, jet, control=None):
+        """write_control_kernel: the double-jet control of `step` written into control."""
+        control = control or self.allocate_control()
+        self._check(self.lib.ds_example_write_control(self.h, int(step), C.byref(jet), _ptr(control["density_source"]),
+                                                      _ptr(control["temperature_source"]), _p3(control["external_acceleration"])),
+                    "ds_example_write_control")
+        return control
+
+    def example_reduce_metrics(self, density, temperature, pre_velocity, post_velocity, out=None):
+        """reduce_metrics_kernel: the six raw metrics as a float64 device tensor."""
+        out = out if out is not None else self._zeros((6,), dtype=self.torch.float64)
+        self._check(self.lib.ds_example_reduce_metrics(self.h, _ptr(density), _ptr(temperature), _p3(pre_velocity),
+                                                       _p3(post_velocity), _ptr(out)), "ds_example_reduce_metrics")
+        return out
 
     # ---- model steps (model.cpp:341-389)
     def forward_step(self, state, control, params, next_state=None, cache=None, check=True):

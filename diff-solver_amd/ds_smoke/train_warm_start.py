@@ -80,6 +80,7 @@ def main(argv=None):
     ap.add_argument("--fp32", action="store_true", help="train without BF16 autocast")
     ap.add_argument("--init", default=None, help="start from these parameters (a file written by this script)")
     ap.add_argument("--solids", type=int, default=0, help="seeded solid spheres in every run (the same placement)")
+    ap.add_argument("--crop", type=int, default=0, help="train on random crop^3 windows of the pairs (0: whole grid)")
     ap.add_argument("--out", required=True)
     args = ap.parse_args(argv)
 
@@ -91,10 +92,11 @@ def main(argv=None):
     print(json.dumps({"phase": "data", "pairs": len(bs), "seconds": round(time.time() - t0, 1)}), flush=True)
     init = warm_start.load(args.init)[0] if args.init else None
     model, output_scale = warm_start.train(bs, ps, unknown, epochs=1000, batch=args.batch, lr=args.lr,
-                                           time_budget=60.0 * args.minutes, log=log, diag=diag, amp=not args.fp32, init=init)
+                                           time_budget=60.0 * args.minutes, log=log, diag=diag, amp=not args.fp32, init=init,
+                                           crop=args.crop)
     flat = warm_start.flatten_parameters(model)
     meta = {"grid": args.grid, "eval_grid": args.eval_grid, "solids": args.solids, "runs": args.runs, "steps": args.steps, "pairs": len(bs),
-            "minutes": args.minutes,
+            "minutes": args.minutes, "crop": args.crop,
             "batch": args.batch, "lr": args.lr, "label_tolerance": 1e-8}
     warm_start.save(args.out, flat, output_scale, meta)
     print(json.dumps({"phase": "train", "output_scale": output_scale, "seconds": round(time.time() - t0, 1)}), flush=True)

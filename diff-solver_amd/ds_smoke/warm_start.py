@@ -115,18 +115,27 @@ def generate_pairs(n: int, runs: int, steps: int, seed: int = 100, jitter_cells:
         if unknown is None:
             code = ctx.masks()[1]
             unknown = torch.from_numpy(code > 0).to(ctx.device)
-        its = 0
-        for _ in range(steps):
-            ctx.forward_step(state, ctl, params, next_state=nxt, cache=cache)
+        dropped = 0
+        for k in range(steps):
+            _, _, info = ctx.forward_step(state, ctl, params, next_state=nxt, cache=cache, check=False)
             ctx.synchronize()
             p = (cache["pressure"][0].double() + cache["pressure"][1].double()).float()
-            b = ctx.apply_matrix(p)
-            ctx.synchronize()
-            bs.append(b.clone())
-            ps.append(p.clone())
+            bad = int((~torch.isfinite(p)).sum())
+            if bad or not info.converged:
+                # a pair whose label is not a converged finite solution would teach the network noise
+                log(f"run {r + 1} step {k}: pair dropped ({bad} non-finite pressure values, "
+                    f"{bad - int((~torch.isfinite(p[unknown])).sum())} of them off the unknowns; converged {bool(info.converged)}, "
+                    f"{info.iterations} iterations)")
+                dropped += 1
+            else:
+                p = torch.where(unknown, p, torch.zeros_like(p))
+                b = ctx.apply_matrix(p)
+                ctx.synchronize()
+                bs.append(b.clone())
+                ps.append(p.clone())
             state, nxt = nxt, state
         ctx.close()
-        log(f"run {r + 1}/{runs}: {len(bs)} pairs")
+        log(f"run {r + 1}/{runs}: {len(bs)} pairs" + (f" ({dropped} dropped)" if dropped else ""))
     return torch.stack(bs), torch.stack(ps), unknown, torch.from_numpy(code.astype(np.float32))
 
 
@@ -163,9 +172,25 @@ def energy_loss(g, b, p, diag):
     return 1.0 - gb * gb / (gag * bp)
 
 
+def crop_pair(p, unknown, diag, crop: int, gen):
+    """A random crop^3 window of a solution p, as an exact pair of the window's own problem:
+    A_w (zero outside the window) applied to the cropped p gives its right-hand side, so the
+    energy loss stays the error's A-norm share for that problem."""
+    import torch
+
+    nz, ny, nx = p.shape
+    o = [int(torch.randint(0, m - crop + 1, (1,), generator=gen)) for m in (nz, ny, nx)]
+    sl = (slice(o[0], o[0] + crop), slice(o[1], o[1] + crop), slice(o[2], o[2] + crop))
+    unk, dg = unknown[sl], diag[0][sl]
+    pc = torch.where(unk, p[sl], torch.zeros_like(p[sl]))
+    return apply_matrix(pc, dg), pc, unk, dg
+
+
 def train(bs, ps, unknown, epochs: int = 20, batch: int = 2, lr: float = 1e-3, seed: int = 0, time_budget: float = 600.0, log=print,
-          diag=None, amp: bool = True, init=None):
-    """Adam on the energy loss of the network's guess (see energy_loss); init: flat parameters to start from."""
+          diag=None, amp: bool = True, init=None, crop: int = 0):
+    """Adam on the energy loss of the network's guess (see energy_loss); init: flat parameters to
+    start from; crop > 0: train on random crop^3 windows of the pairs (crop_pair) instead of the
+    whole grid."""
     import torch
 
     torch.manual_seed(seed)
@@ -179,9 +204,11 @@ def train(bs, ps, unknown, epochs: int = 20, batch: int = 2, lr: float = 1e-3, s
         ratios = []
         for i in range(0, len(bs), max(1, len(bs) // 16)):
             _, s = network_input(bs[i], unknown)
-            ratios.append(float(ps[i].std() / s))
-        output_scale = float(np.median(ratios)) or 1.0
+            ratios.append(float(ps[i][unknown].std() / s))
+        ratios = [v for v in ratios if np.isfinite(v) and v > 0.0]
+        output_scale = float(np.median(ratios)) if ratios else 1.0
     opt = torch.optim.Adam(model.parameters(), lr=lr)
+    gen = torch.Generator().manual_seed(seed)
     n = len(bs)
     t0 = time.time()
     step = 0
@@ -190,7 +217,14 @@ def train(bs, ps, unknown, epochs: int = 20, batch: int = 2, lr: float = 1e-3, s
         total = 0.0
         for i in range(0, n, batch):
             idx = perm[i:i + batch]
-            xs, ss = zip(*[network_input(bs[j], unknown) for j in idx.tolist()])
+            if crop:
+                wins = [crop_pair(ps[j], unknown, diag, crop, gen) for j in idx.tolist()]
+                bw, pw = [w[0] for w in wins], [w[1] for w in wins]
+                unk_b, diag_b = torch.stack([w[2] for w in wins]), torch.stack([w[3] for w in wins])
+            else:
+                bw, pw = [bs[j] for j in idx.tolist()], [ps[j] for j in idx.tolist()]
+                unk_b, diag_b = unknown[None], diag
+            xs, ss = zip(*[network_input(b, u) for b, u in zip(bw, unk_b if crop else [unknown] * len(bw))])
             x = torch.cat(xs).contiguous(memory_format=torch.channels_last_3d)
             try:
                 with torch.autocast("cuda", dtype=torch.bfloat16, enabled=amp):
@@ -201,10 +235,10 @@ def train(bs, ps, unknown, epochs: int = 20, batch: int = 2, lr: float = 1e-3, s
                 log(f"BF16 autocast failed ({e}); training in FP32")
                 amp = False
                 out = model(x)
-            g = out.float()[:, 0] * unknown[None].float()
-            bn = torch.stack([bs[j] / float(s) for j, s in zip(idx.tolist(), ss)])
-            pn = torch.stack([ps[j] / float(s) for j, s in zip(idx.tolist(), ss)])
-            loss = energy_loss(g, bn, pn, diag).mean()
+            g = out.float()[:, 0] * unk_b.float()
+            bn = torch.stack([b / float(s) for b, s in zip(bw, ss)])
+            pn = torch.stack([p / float(s) for p, s in zip(pw, ss)])
+            loss = energy_loss(g, bn, pn, diag_b).mean()
             opt.zero_grad()
             loss.backward()
             opt.step()

@@ -427,6 +427,37 @@ ds_status ds_trajectory_jvp(ds_context* c, const ds_state* initial, const ds_con
                             const ds_parameter_tangent* param_tangent, ds_state* final_state,
                             ds_state_tangent* final_tangent, ds_trajectory_info* info);
 
+/* ---- forward example harness (examples/smoke/forward/simulation.{ixx,cpp,cu}) ------------- */
+
+/* The control arguments of launch_write_control (simulation.h:10, ForwardSimulationOptions at
+ * simulation.ixx:9-27): two Gaussian jets (centres and radius in domain units) whose amplitudes
+ * 0.75 + 0.25 sin(2 pi t / pulse_period [+ pi]) pulse in antiphase. */
+typedef struct ds_jet_control {
+    float pulse_period;
+    float left_source_center[3];
+    float right_source_center[3];
+    float source_radius;
+    float density_source_rate;
+    float temperature_source_rate;
+    float left_acceleration[3];
+    float right_acceleration[3];
+} ds_jet_control;
+
+/* write_control_kernel (simulation.cu:35-57) for step `step` (time step * dt): overwrites the
+ * density and temperature sources (rate * summed jet weight) and the cell-centred external
+ * acceleration (each jet's weight times its vector) on every cell. */
+ds_status ds_example_write_control(ds_context* c, uint64_t step, const ds_jet_control* jet, float* density_source,
+                                   float* temperature_source, float* const external_acceleration[3]);
+
+/* reduce_metrics_kernel (simulation.cu:59-85) over the fluid cells: metrics (DEVICE, 6 doubles,
+ * overwritten) = { sum rho, max rho, max T, max |cell-centred post-projection velocity|,
+ * sum div(pre)^2, sum div(post)^2 }, divergence = face differences / h. The maxima are taken
+ * with 0 (the reference zeroes the buffer first). Deterministic: a fixed-order two-pass
+ * reduction (the reference uses FP64 atomics, simulation.cu:78-83). */
+ds_status ds_example_reduce_metrics(ds_context* c, const float* density, const float* temperature,
+                                    const float* const pre_projection_velocity[3],
+                                    const float* const post_projection_velocity[3], double* metrics);
+
 #ifdef __cplusplus
 }
 #endif

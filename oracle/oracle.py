@@ -61,6 +61,10 @@ def lib():
         L.oracle_vorticity_jvp.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_void_p]
         for name in ("oracle_projection_jvp", "oracle_jvp_step", "oracle_trajectory_jvp"):
             getattr(L, name).restype = C.c_int
+        L.oracle_example_write_control.restype = None
+        L.oracle_example_write_control.argtypes = [C.c_void_p, C.c_uint64, DP, DP, DP, C.c_void_p]
+        L.oracle_example_reduce_metrics.restype = None
+        L.oracle_example_reduce_metrics.argtypes = [C.c_void_p, DP, DP, C.c_void_p, C.c_void_p, DP]
         L.oracle_set_threads.restype = C.c_int
         L.oracle_set_threads.argtypes = [C.c_int]
         _lib = L
@@ -385,6 +389,24 @@ class Oracle:
                "params": pa, "info": info}
         if ca is not None:
             out["control"] = {"density_source": ca[0], "temperature_source": ca[1], "external_acceleration": tuple(ca[2:5])}
+        return out
+
+    # -- forward example harness (examples/smoke/forward/simulation.cu)
+    def example_write_control(self, step, jet):
+        """(density source, temperature source, acceleration) of the double jet at `step`; jet: the
+        16 values of ds_jet_control in its field order."""
+        s_rho, s_t = self.zeros_cell(), self.zeros_cell()
+        e = tuple(self.zeros_cell() for _ in range(3))
+        j = f64(jet)
+        assert j.size == 16
+        lib().oracle_example_write_control(C.c_void_p(self.h), int(step), _p(j), _p(s_rho), _p(s_t), _arr3(e))
+        return s_rho, s_t, e
+
+    def example_reduce_metrics(self, rho, T, pre, post):
+        """The six raw metrics of reduce_metrics_kernel over the fluid cells."""
+        out = np.zeros(6)
+        lib().oracle_example_reduce_metrics(C.c_void_p(self.h), _p(f64(rho)), _p(f64(T)), _arr3([f64(v) for v in pre]),
+                                            _arr3([f64(v) for v in post]), _p(out))
         return out
 
     def simulate(self, state, control, params, steps):

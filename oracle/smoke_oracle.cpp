@@ -1638,6 +1638,80 @@ int build(Oracle& o, const ds_config& cfg) {
     return DS_OK;
 }
 
+// ---- forward example harness (examples/smoke/forward/simulation.cu) -------------------------------
+struct Jet {
+    double pulse_period;
+    double lc[3], rc[3];
+    double radius, rho_rate, t_rate;
+    double la[3], ra[3];
+};
+
+// write_control_kernel, simulation.cu:35-57 (gaussian :24-29), in FP64: two Gaussian jets with
+// amplitudes 0.75 + 0.25 sin(2 pi t / T) and 0.75 + 0.25 sin(2 pi t / T + pi), t = step * dt.
+void example_write_control(const Oracle& o, std::uint64_t step, const Jet& j, double* s_rho, double* s_t, double* const e[3]) {
+    const double two_pi = 6.28318530717958647692;
+    const double time = (double)step * (double)o.dt;
+    const double phase = two_pi * time / j.pulse_period;
+    const double amp_l = 0.75 + 0.25 * std::sin(phase);
+    const double amp_r = 0.75 + 0.25 * std::sin(phase + 0.5 * two_pi);
+    auto gauss = [&](double x, double y, double z, const double c[3]) {
+        const double dx = x - c[0], dy = y - c[1], dz = z - c[2];
+        return std::exp(-0.5 * (dx * dx + dy * dy + dz * dz) / (j.radius * j.radius));
+    };
+    for (int z = 0; z < o.nz; ++z)
+        for (int y = 0; y < o.ny; ++y)
+            for (int x = 0; x < o.nx; ++x) {
+                const i64 i = x + (i64)o.nx * (y + (i64)o.ny * z);
+                const double px = (x + 0.5) * o.h, py = (y + 0.5) * o.h, pz = (z + 0.5) * o.h;
+                const double wl = amp_l * gauss(px, py, pz, j.lc), wr = amp_r * gauss(px, py, pz, j.rc);
+                s_rho[i] = j.rho_rate * (wl + wr);
+                s_t[i] = j.t_rate * (wl + wr);
+                for (int a = 0; a < 3; ++a) e[a][i] = wl * j.la[a] + wr * j.ra[a];
+            }
+}
+
+// reduce_metrics_kernel, simulation.cu:59-85, over fluid cells (mask == 0): out = { sum rho,
+// max(0, rho), max(0, T), max(0, |cell-centred post velocity|), sum div_pre^2, sum div_post^2 }.
+// The fields are FP32 in the reference, so each face value is taken as float, the face pair of
+// the cell velocity is summed in FP32 and halved in FP64, and the divergence is the FP32 sum
+// x1 - x0 + y1 - y0 + z1 - z0 in that order, scaled by 1/h (h as float) in FP64.
+void example_reduce_metrics(const Oracle& o, const double* rho, const double* T, const double* const pre[3], const double* const post[3],
+                            double out[6]) {
+    for (int k = 0; k < 6; ++k) out[k] = 0.0;
+    const double inv_h = 1.0 / (double)o.hf;
+    for (int z = 0; z < o.nz; ++z)
+        for (int y = 0; y < o.ny; ++y)
+            for (int x = 0; x < o.nx; ++x) {
+                const i64 i = x + (i64)o.nx * (y + (i64)o.ny * z);
+                if (o.mask[i] != 0) continue;
+                const i64 x0 = x + (i64)(o.nx + 1) * (y + (i64)o.ny * z), x1 = x0 + 1;
+                const i64 y0 = x + (i64)o.nx * (y + (i64)(o.ny + 1) * z), y1 = y0 + o.nx;
+                const i64 z0 = i, z1 = i + (i64)o.nx * o.ny;
+                auto f = [](const double* a, i64 k) { return (float)a[k]; };
+                const double vx = 0.5 * (double)(f(post[0], x0) + f(post[0], x1));
+                const double vy = 0.5 * (double)(f(post[1], y0) + f(post[1], y1));
+                const double vz = 0.5 * (double)(f(post[2], z0) + f(post[2], z1));
+                float dpre = f(pre[0], x1) - f(pre[0], x0);
+                dpre = dpre + f(pre[1], y1);
+                dpre = dpre - f(pre[1], y0);
+                dpre = dpre + f(pre[2], z1);
+                dpre = dpre - f(pre[2], z0);
+                float dpost = f(post[0], x1) - f(post[0], x0);
+                dpost = dpost + f(post[1], y1);
+                dpost = dpost - f(post[1], y0);
+                dpost = dpost + f(post[2], z1);
+                dpost = dpost - f(post[2], z0);
+                const double dv_pre = (double)dpre * inv_h, dv_post = (double)dpost * inv_h;
+                const double speed = std::sqrt(vx * vx + vy * vy + vz * vz);
+                out[0] += (double)(float)rho[i];
+                if ((double)(float)rho[i] > out[1]) out[1] = (double)(float)rho[i];
+                if ((double)(float)T[i] > out[2]) out[2] = (double)(float)T[i];
+                if (speed > out[3]) out[3] = speed;
+                out[4] += dv_pre * dv_pre;
+                out[5] += dv_post * dv_post;
+            }
+}
+
 }  // namespace
 
 // ---- C API for ctypes (test infrastructure) ---------------------------------------------------
@@ -1810,6 +1884,27 @@ int oracle_loss_gradient(void* h, const double* const init[5], const double* con
                          ds_trajectory_info* info) {
     return loss_gradient(*static_cast<Oracle*>(h), init, control, params, steps, k, target, final_state, loss, init_adj,
                          control_adj, param_adj, info);
+}
+
+// jet: pulse_period, left centre[3], right centre[3], radius, density rate, temperature rate,
+// left acceleration[3], right acceleration[3] (16 values, the order of ds_jet_control)
+void oracle_example_write_control(void* h, std::uint64_t step, const double* jet, double* s_rho, double* s_t, double* const e[3]) {
+    Jet j{};
+    j.pulse_period = jet[0];
+    for (int a = 0; a < 3; ++a) {
+        j.lc[a] = jet[1 + a];
+        j.rc[a] = jet[4 + a];
+        j.la[a] = jet[10 + a];
+        j.ra[a] = jet[13 + a];
+    }
+    j.radius = jet[7];
+    j.rho_rate = jet[8];
+    j.t_rate = jet[9];
+    example_write_control(*static_cast<Oracle*>(h), step, j, s_rho, s_t, e);
+}
+void oracle_example_reduce_metrics(void* h, const double* rho, const double* T, const double* const pre[3], const double* const post[3],
+                                   double* out) {
+    example_reduce_metrics(*static_cast<Oracle*>(h), rho, T, pre, post, out);
 }
 
 }  // extern "C"

@@ -21,6 +21,7 @@ MIRRORS = {
     "ds_parameter_adjoint": abi.ParameterAdjoint, "ds_solve_info": abi.SolveInfo,
     "ds_step_info": abi.StepInfo, "ds_trajectory_info": abi.TrajectoryInfo,
     "ds_kernel_timing": abi.KernelTiming, "ds_domain_info": abi.DomainInfo,
+    "ds_jet_control": abi.JetControl,
 }
 
 

@@ -1,0 +1,202 @@
+"""The forward example harness on the GPU (examples/smoke/forward/: simulation.cu:35-85,
+simulation.cpp:61-120, main.cpp): ds_example_write_control and ds_example_reduce_metrics
+against the oracle, ForwardSimulation (RBGS, the reference's projection) against the oracle's
+forward steps, the C++ program against the Python harness, and the full 128 x 192 x 128 set-up.
+
+Bars: the control is FP32 with expf / sinf against FP64, relative 2e-6; the metric sums are
+FP64 over the same FP32 terms in another order, relative 1e-12, the maxima of the FP32 fields are
+exact; the simulation after 12 steps, relative L2 1e-4 on density against FP64."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from helpers import rel, scene_solids
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+PROGRAM = os.path.join(ROOT, "diff-solver_amd", "lib", "smoke_forward")
+SMALL = dict(resolution=(32, 48, 32), cell_size=1.0 / 32.0)
+
+
+def _small_options(**kw):
+    from ds_smoke.example import ForwardSimulationOptions
+
+    return ForwardSimulationOptions(**SMALL, **kw)
+
+
+def test_write_control_matches_oracle():
+    import oracle as O
+    from ds_smoke import abi
+    from ds_smoke.solver import Context
+
+    opts = _small_options()
+    print("write_control 32x48x32", flush=True)
+    ctx = Context(opts.config(), abi.MODE_FORWARD)
+    orc = O.Oracle(opts.scene(), solver=abi.PROJECTION_RBGS, rbgs_iterations=4)
+    jet = opts.jet()
+    ctl = ctx.allocate_control()
+    for step in (0, 1, 27, 108, 959):
+        ctx.example_write_control(step, jet, ctl)
+        got = ctx.to_host(ctl)
+        s_rho, s_t, e = orc.example_write_control(step, jet.values())
+        for name, g, r in (("density", got["density_source"], s_rho), ("temperature", got["temperature_source"], s_t)) + tuple(
+                (f"accel{a}", got["external_acceleration"][a], e[a]) for a in range(3)):
+            err = rel(g, r)
+            assert err < 2e-6, (step, name, err)
+            assert np.abs(g - r).max() <= 2e-6 * np.abs(r).max(), (step, name)
+    ctx.close()
+
+
+@pytest.fixture(scope="module")
+def solids_ctx():
+    import oracle as O
+    from ds_smoke import abi
+    from ds_smoke.solver import Context
+
+    sc = scene_solids()
+    return sc, Context(sc.config(), abi.MODE_FORWARD), O.Oracle(sc)
+
+
+def test_reduce_metrics_matches_oracle_and_is_deterministic(solids_ctx):
+    from ds_smoke import scenes
+
+    sc, ctx, orc = solids_ctx
+    print("reduce_metrics solids scene", flush=True)
+    rng = np.random.default_rng(11)
+    rho = rng.standard_normal(scenes.cell_shape(sc.resolution)).astype(np.float32)
+    T = rng.standard_normal(scenes.cell_shape(sc.resolution)).astype(np.float32)
+    pre = tuple(rng.standard_normal(scenes.face_shape(sc.resolution, a)).astype(np.float32) for a in range(3))
+    post = tuple(rng.standard_normal(scenes.face_shape(sc.resolution, a)).astype(np.float32) for a in range(3))
+    d = [ctx.to_device(x) for x in (rho, T)]
+    dpre, dpost = tuple(ctx.to_device(x) for x in pre), tuple(ctx.to_device(x) for x in post)
+    out = ctx.to_host(ctx.example_reduce_metrics(d[0], d[1], dpre, dpost))
+    ref = orc.example_reduce_metrics(rho, T, pre, post)
+    print(f"  gpu {out}\n  ref {ref}")
+    np.testing.assert_allclose(out[[0, 4, 5]], ref[[0, 4, 5]], rtol=1e-12)
+    assert out[1] == ref[1] and out[2] == ref[2]
+    np.testing.assert_allclose(out[3], ref[3], rtol=1e-14)
+    again = ctx.to_host(ctx.example_reduce_metrics(d[0], d[1], dpre, dpost))
+    np.testing.assert_array_equal(again, out)
+    # maxima taken with 0, solid cells left out
+    mask = ctx.masks()[0]
+    neg = np.where(mask == 0, -1.0, 1e6).astype(np.float32)
+    out = ctx.to_host(ctx.example_reduce_metrics(ctx.to_device(neg), ctx.to_device(neg), dpre, dpost))
+    assert out[0] == -float((mask == 0).sum()) and out[1] == 0.0 and out[2] == 0.0
+
+
+def _oracle_run(opts, steps):
+    """The harness restated on the oracle: control of step k, one FP64 forward step, metrics."""
+    import oracle as O
+    from ds_smoke import abi, scenes
+
+    sc = opts.scene()
+    orc = O.Oracle(sc, solver=opts.projection_solver, rbgs_iterations=opts.pressure_iterations, tolerance=1e-10)
+    state = {"density": np.zeros(scenes.cell_shape(sc.resolution)), "temperature": np.zeros(scenes.cell_shape(sc.resolution)),
+             "velocity": tuple(np.zeros(scenes.face_shape(sc.resolution, a)) for a in range(3))}
+    jet = opts.jet().values()
+    raw = []
+    for k in range(steps):
+        s_rho, s_t, e = orc.example_write_control(k, jet)
+        ctl = {"density_source": s_rho, "temperature_source": s_t, "external_acceleration": e}
+        state, cache, _ = orc.forward_step(state, ctl, np.array(sc.params))
+        raw.append(orc.example_reduce_metrics(state["density"], state["temperature"], cache["advected_velocity"], state["velocity"]))
+    assert abi.PROJECTION_RBGS == opts.projection_solver
+    return state, raw
+
+
+def test_forward_simulation_matches_oracle():
+    from ds_smoke.example import ForwardSimulation, ForwardSimulationMetrics, finish_metrics
+
+    opts = _small_options()
+    steps = 12
+    print(f"ForwardSimulation 32x48x32 RBGS {opts.pressure_iterations}, {steps} steps", flush=True)
+    sim = ForwardSimulation(opts)
+    history = []
+    for _ in range(steps):
+        sim.step()
+        history.append(dict(vars(sim.metrics)))
+    ref_state, ref_raw = _oracle_run(opts, steps)
+    got = sim.context.to_host(sim.current_state)
+    err = rel(got["density"], ref_state["density"])
+    print(f"  density after {steps} steps: relative L2 {err:.2e}")
+    assert err < 1e-4, err
+    for a in range(3):
+        assert rel(got["velocity"][a], ref_state["velocity"][a]) < 1e-4
+    m_ref = ForwardSimulationMetrics()
+    for k, raw in enumerate(ref_raw):
+        m_ref.step = k + 1
+        finish_metrics(opts, raw, 0.0, m_ref)
+        m = history[k]
+        for key, tol in (("density_mass", 1e-5), ("density_maximum", 1e-4), ("temperature_maximum", 1e-4),
+                         ("maximum_velocity", 1e-4), ("pre_projection_divergence_rms", 1e-3),
+                         ("post_projection_divergence_rms", 1e-3)):
+            assert abs(m[key] - getattr(m_ref, key)) <= tol * abs(getattr(m_ref, key)), (k, key, m[key], getattr(m_ref, key))
+    last = history[-1]
+    assert last["step"] == steps and last["density_mass"] > 0.0 and 0.0 < last["divergence_ratio"] < 1.0
+    # reset gives the same run again, bit for bit
+    sim.reset()
+    for _ in range(steps):
+        sim.step()
+    np.testing.assert_array_equal(sim.context.to_host(sim.current_state)["density"], got["density"])
+    assert sim.metrics.density_mass == last["density_mass"]
+    sim.close()
+
+
+def test_cpp_program_matches_python_harness(tmp_path):
+    from ds_smoke.example import ForwardSimulation
+
+    if not os.path.exists(PROGRAM):
+        pytest.fail(f"{PROGRAM} is missing: run __graft_entry__.build() first")
+    steps = 12
+    print(f"smoke_forward 32x48x32, {steps} steps", flush=True)
+    out = subprocess.run([PROGRAM, "--steps", str(steps), "--report", "6", "--resolution", "32", "48", "32", "--cell-size", "0.03125",
+                          "--dump", str(tmp_path)], capture_output=True, text=True, timeout=120)
+    print(out.stdout)
+    assert out.returncode == 0, out.stderr
+    assert "double-jet thermal smoke" in out.stdout and out.stdout.count("step ") >= 2
+    sim = ForwardSimulation(_small_options())
+    rows = []
+    for _ in range(steps):
+        sim.step()
+        rows.append(dict(vars(sim.metrics)))
+    final = np.fromfile(tmp_path / "final_density.f32", dtype=np.float32)
+    np.testing.assert_array_equal(final, sim.context.to_host(sim.current_state)["density"].ravel())
+    csv = np.genfromtxt(tmp_path / "metrics.csv", delimiter=",", names=True)
+    assert len(csv) == steps
+    for k in range(steps):
+        for key in ("density_mass", "density_maximum", "temperature_maximum", "maximum_velocity", "cfl",
+                    "pre_projection_divergence_rms", "post_projection_divergence_rms"):
+            assert csv[key][k] == rows[k][key], (k, key)
+    sim.close()
+
+
+def test_pcg_projection_and_full_size():
+    """PCG to 1e-6 removes most of the divergence the fixed RBGS sweeps leave (what remains sits
+    on the open top, whose faces the pressure equation does not constrain); the full
+    128 x 192 x 128 set-up runs its first 60 steps (semi-Lagrangian advection: the jets reach a
+    CFL number above 1 by then, as in the reference) with finite metrics and positive mass."""
+    from ds_smoke import abi
+    from ds_smoke.example import ForwardSimulation, ForwardSimulationOptions
+
+    print("ForwardSimulation 32x48x32 PCG, 12 steps", flush=True)
+    pcg = ForwardSimulation(_small_options(projection_solver=abi.PROJECTION_PCG))
+    rbgs = ForwardSimulation(_small_options())
+    for _ in range(12):
+        pcg.step()
+        rbgs.step()
+    print(f"  divergence ratio: PCG {pcg.metrics.divergence_ratio:.3e}, RBGS {rbgs.metrics.divergence_ratio:.3e}")
+    assert pcg.metrics.divergence_ratio < 0.1 * rbgs.metrics.divergence_ratio
+    pcg.close()
+    rbgs.close()
+    print("ForwardSimulation 128x192x128 RBGS 160, 60 steps", flush=True)
+    sim = ForwardSimulation(ForwardSimulationOptions())
+    for _ in range(60):
+        sim.step()
+    m = sim.metrics
+    print(f"  step {m.step}: mass {m.density_mass:.6f}, CFL {m.cfl:.4f}, divergence ratio {m.divergence_ratio:.4f}, "
+          f"{m.average_step_milliseconds:.3f} ms/step")
+    assert m.step == 60 and 0.0 < m.cfl < 5.0 and m.density_mass > 0.0 and m.divergence_ratio < 1.0
+    sim.close()
