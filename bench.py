@@ -32,6 +32,10 @@ HBM_PEAK_GBS = 8000.0
 BYTES_UNKNOWN = {"pcg_direction_kernel": 4 + 4 + 1 + 4, "pcg_update_kernel": 4 + 4 + 1 + 8 + 4 + 8,
                  "pcg_fused_kernel": 1 + 4 * 4 + 4 + 4 * 4 + 4}
 BYTES_OTHER = {"pcg_direction_kernel": 1 + 4, "pcg_update_kernel": 1, "pcg_fused_kernel": 1 + 4 + 4}
+# the resident kernel performs the streamed kernel's iteration with its vectors held on chip: the
+# algorithmic bytes of an iteration are the same (its timing unit is one iteration)
+BYTES_UNKNOWN["pcg_resident_kernel"] = BYTES_UNKNOWN["pcg_fused_kernel"]
+BYTES_OTHER["pcg_resident_kernel"] = BYTES_OTHER["pcg_fused_kernel"]
 
 
 def host_threads():
@@ -111,8 +115,10 @@ def main():
                          "network; all = each in turn (the network when its weights exist), value = the fastest")
     ap.add_argument("--weights", default=os.path.join(ROOT, "diff-solver_amd", "data", "warm_start_plume.npz"),
                     help="warm-start network parameters (for --warm-start cnn)")
-    ap.add_argument("--cg-variant", choices=["single", "two"], default="single",
-                    help="PCG recurrence: single-reduction (one fused kernel per iteration) or two-reduction")
+    ap.add_argument("--cg-variant", choices=["single", "two", "resident"], default="single",
+                    help="PCG recurrence: single-reduction (one fused kernel per iteration), resident (the same "
+                         "recurrence as one cooperative launch per replacement interval where the grid allows) "
+                         "or two-reduction")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 10 device-to-device copies of MIB MiB (the reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -128,7 +134,8 @@ def main():
     from ds_smoke.solver import Context
 
     sc = scenes.plume(args.grid)
-    variant = {"single": abi.CG_SINGLE_REDUCTION, "two": abi.CG_TWO_REDUCTION}[args.cg_variant]
+    variant = {"single": abi.CG_SINGLE_REDUCTION, "two": abi.CG_TWO_REDUCTION,
+               "resident": abi.CG_SINGLE_REDUCTION_RESIDENT}[args.cg_variant]
     if args.warm_start == "all":
         modes = ["none", "previous"] + (["cnn"] if os.path.exists(args.weights) else [])
     else:

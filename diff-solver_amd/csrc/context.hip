@@ -98,7 +98,7 @@ void validate(const ds_config& cfg, uint32_t mode) {
     if (cfg.warm_start == DS_WARM_START_CNN && (cfg.resolution[0] % 64 || cfg.resolution[1] % 16 || cfg.resolution[2] % 4))
         throw Failure{DS_ERROR_UNSUPPORTED, "the warm-start network needs nx % 64 == 0, ny % 16 == 0 and nz % 4 == 0"};
     if (!(cfg.cg_tolerance >= 0.0) || !std::isfinite(cfg.cg_tolerance)) bad("cg_tolerance must be >= 0 and finite");
-    if (cfg.cg_variant > DS_CG_TWO_REDUCTION) bad("cg_variant out of range");
+    if (cfg.cg_variant > DS_CG_SINGLE_REDUCTION_RESIDENT) bad("cg_variant out of range");
 }
 
 // model.cpp:56-182 plus the pressure connectivity (operators.cu:852-873): solid mask from cell
@@ -400,7 +400,8 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
         c->pcg.p[0] = device_alloc<float>(c, n + 1);
         c->pcg.p[1] = device_alloc<float>(c, n + 1);
         for (float* v : {c->pcg.r, c->pcg.p[0], c->pcg.p[1]}) fill_zero(c, v, n + 1);
-        c->pcg.variant = cfg.cg_variant;
+        // the resident single-reduction variant runs the same recurrence
+        c->pcg.variant = cfg.cg_variant == DS_CG_TWO_REDUCTION ? DS_CG_TWO_REDUCTION : DS_CG_SINGLE_REDUCTION;
         if (c->pcg.variant == DS_CG_SINGLE_REDUCTION) {
             // s is written on unknowns only; its other cells must read as 0 (as halo values)
             c->pcg.r_alt = device_alloc<float>(c, n + 1);
@@ -434,6 +435,7 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
         c->pcg.partials = device_alloc<double>(c, 2 * (size_t)kPartCount * kPcgMaxBlocks);   // two sets
         c->pcg.hist = device_alloc<double>(c, 2 * ((size_t)c->max_iterations + 4));
         c->pcg.status = device_alloc<PcgStatus>(c, 1);
+        if (cfg.cg_variant == DS_CG_SINGLE_REDUCTION_RESIDENT) pcg_resident_prepare(c);
         c->metric_partials = device_alloc<double>(c, 6 * (size_t)kMetricBlocks);
         void* hs = nullptr;
         DS_HIP(hipHostMalloc(&hs, 2 * sizeof(PcgStatus), hipHostMallocDefault));

@@ -88,14 +88,14 @@ struct PcgStatus {
     uint32_t stop;          // 1 once converged (or the guess made the residual small enough)
     uint32_t iterations;    // iterations performed when stop was set
     uint32_t rejected;
-    uint32_t pad;
+    uint32_t aborted;       // 1 when a resident launch's grid barrier exceeded its bound
     double bb;              // ||b||^2
     double rr;              // last reduced ||r||^2 (recursive or true)
 };
 
 // HIP-event timing of sampled PCG kernel launches (kernel 0 = direction, 1 = update, 2 = the
 // fused single-reduction iteration).
-constexpr int kProfiledKernels = 3;
+constexpr int kProfiledKernels = 4;   // 3 = the resident single-reduction launch (units: iterations)
 struct Profiler {
     uint32_t every = 0;
     uint64_t launches[kProfiledKernels]{};
@@ -104,6 +104,7 @@ struct Profiler {
     struct Rec {
         int kernel;
         hipEvent_t a, b;
+        uint32_t units = 1;   // iterations the timed launch performed (resident launches)
     };
     std::vector<Rec> recs;
     hipEvent_t take() {
@@ -142,6 +143,10 @@ struct PcgWork {
     PcgStatus* status = nullptr; // device
     PcgStatus* host_status = nullptr;  // pinned, 2 slots (double-buffered batches)
     hipEvent_t batch_event[2]{};
+    // single-reduction form run as one cooperative launch per replacement interval with the
+    // vectors on chip (pcg_resident.hpp; DS_CG_SINGLE_REDUCTION_RESIDENT), when the grid allows it
+    bool resident = false;
+    uint32_t* resident_sync = nullptr;   // ResidentSync (2 words)
 };
 
 }  // namespace ds
@@ -257,6 +262,9 @@ ds_solve_info pcg_solve(ds_context* c, float* x_hi, float* x_lo, const float* x0
 ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], const float* const extra[3], float* xh,
                              float* xl, const float* x0h, const float* x0l, double tol, uint32_t max_it);
 uint32_t pcg_periodic_bits(const ds_context* c);
+// Decides whether the single-reduction solve runs resident (full 64 x kTY tiles, at most 8 planes
+// per block, cooperative launch possible with every block co-resident) and allocates its state.
+void pcg_resident_prepare(ds_context* c);
 // DS_PROJECTION_RBGS (rbgs.hip): the diagonal table, then the solve of a right-hand-side kind
 // (1 forward, 2 the unrolled reverse, 3 tangent) into xh with xl = 0.
 void rbgs_prepare(ds_context* c);

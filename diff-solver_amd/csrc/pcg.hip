@@ -28,6 +28,7 @@
 #include "context.hpp"
 #include "pcg_fused.hpp"
 #include "pcg_plane.hpp"
+#include "pcg_resident.hpp"
 #include "rhs.hpp"
 
 namespace ds {
@@ -136,6 +137,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_setup_kernel(Geo g, uint32_t 
             status->stop = 0u;
             status->iterations = 0u;
             status->rejected = 0u;
+            status->aborted = 0u;
         }
     }
 }
@@ -310,6 +312,27 @@ void launch_fused(ds_context* c, uint32_t per, const FusedVecs& v, const double*
                                                                     w.hist, w.status, tol2, k, check_only, respect_stop, with_ps);
 }
 
+// One cooperative launch of iterations k0 .. k0+count-1 of the single-reduction form, reading set
+// `set` (every block must be co-resident: pcg_resident_prepare checked the occupancy).
+void launch_resident(ds_context* c, uint32_t per, int set, double* const parts[2], const double* bb_parts, double tol2, uint32_t k0,
+                     uint32_t count) {
+    PcgWork& w = c->pcg;
+    fill_zero(c, reinterpret_cast<float*>(w.resident_sync), 2);
+    Geo g = c->geo;
+    PlaneTiling pt = w.tiling;
+    const uint8_t* code = c->code;
+    ResidentVecs v{w.r, w.r_alt, w.wv[0], w.wv[1], w.sv[0], w.sv[1], w.p[0], w.xd};
+    double* p0 = parts[0];
+    double* p1 = parts[1];
+    double* hist = w.hist;
+    PcgStatus* status = w.status;
+    ResidentSync* sync = reinterpret_cast<ResidentSync*>(w.resident_sync);
+    uint32_t set0 = (uint32_t)set;
+    void* args[]{&g, &per, &pt, &code, &v, &p0, &p1, &bb_parts, &hist, &status, &sync, &tol2, &k0, &count, &set0};
+    DS_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void*>(&pcg_resident_kernel<kResidentZ>), dim3(w.blocks), dim3(kPcgThreads),
+                                      args, 0, c->stream));
+}
+
 // Single-reduction iteration from the state left by launch_setup (r in set 0, BB/RR/RZ partials
 // in partial set 0) until the true residual meets tol or max_it.
 ds_solve_info pcg_iterate_fused(ds_context* c, float* xh, float* xl, double tol, uint32_t max_it, bool warm, bool rejected) {
@@ -356,12 +379,40 @@ ds_solve_info pcg_iterate_fused(ds_context* c, float* xh, float* xl, double tol,
     bool first = true;
     uint32_t restarts = 0;
     Profiler& prof = c->profiler;
+    std::vector<std::pair<size_t, uint32_t>> resident_recs;   // (profiler record, first iteration)
     while (true) {
         uint32_t batch = 1;
         int slot = 0, pending = -1;
         bool stop_seen = false;
         while (k < max_it && !stop_seen) {
             const uint32_t nb = std::min(batch, max_it - k);
+            if (w.resident) {
+                // one cooperative launch per replacement interval; `batch` of them per status check
+                for (uint32_t j = 0; j < nb && k < max_it; ++j) {
+                    const uint32_t count = std::min(R - k % R, max_it - k);
+                    hipEvent_t e0 = nullptr, e1 = nullptr;
+                    if (prof.every && prof.used + 2 <= 6000) {
+                        e0 = prof.take();
+                        e1 = prof.take();
+                    }
+                    if (e0) DS_HIP(hipEventRecord(e0, c->stream));
+                    launch_resident(c, per, set, parts, first ? parts[0] : nullptr, tol2, k, count);
+                    first = false;
+                    if (e1) {
+                        DS_HIP(hipEventRecord(e1, c->stream));
+                        prof.recs.push_back({3, e0, e1, count});
+                        resident_recs.push_back({prof.recs.size() - 1, k});
+                    }
+                    if (prof.every) prof.launches[3] += count;
+                    k += count;
+                    if (count & 1u) set ^= 1;
+                    if (k % R == 0u && k < max_it) {
+                        replace(set, true, 1u);
+                        prime(set, 1u, true);
+                    }
+                }
+                check_launch("pcg_resident_kernel");
+            } else
             for (uint32_t j = 0; j < nb; ++j, ++k) {
                 hipEvent_t e0 = nullptr, e1 = nullptr;
                 if (prof.every && k % prof.every == 0 && prof.used + 2 <= 6000) {
@@ -399,6 +450,12 @@ ds_solve_info pcg_iterate_fused(ds_context* c, float* xh, float* xl, double tol,
             batch = std::min<uint32_t>(batch * 2u, 32u);
         }
         PcgStatus st = read_status(c);
+        if (st.aborted) {
+            // a grid barrier wait ran past its bound: the vectors are incomplete. Later solves of
+            // this context use one launch per iteration.
+            w.resident = false;
+            throw Failure{DS_ERROR_RUNTIME, "pcg_resident_kernel: a grid barrier wait exceeded its bound"};
+        }
         if (!st.stop && k >= max_it) {
             // one more convergence test on the last iteration's residual
             if (k == 0)
@@ -414,6 +471,11 @@ ds_solve_info pcg_iterate_fused(ds_context* c, float* xh, float* xl, double tol,
         const uint32_t done = st.stop ? st.iterations : max_it;
         set = (int)((uint32_t)set ^ ((k - done) & 1u));
         k = done;
+        for (const auto& rr : resident_recs) {   // the iterations each timed resident launch performed
+            Profiler::Rec& rec = prof.recs[rr.first];
+            rec.units = done > rr.second ? std::min(rec.units, done - rr.second) : 0u;
+        }
+        resident_recs.clear();
         replace(set, done > 0, 0u);
         pcg_finalize_kernel<<<1, kBlock, 0, c->stream>>>(parts[set], w.blocks, w.status, 0u);
         check_launch("pcg_fused_replace_kernel");
@@ -527,6 +589,23 @@ ds_solve_info pcg_iterate(ds_context* c, float* xh, float* xl, double tol, uint3
 
 uint32_t pcg_periodic_bits(const ds_context* c) { return periodic_bits(c); }
 
+void pcg_resident_prepare(ds_context* c) {
+    PcgWork& w = c->pcg;
+    w.resident = false;
+    const Geo& g = c->geo;
+    if (g.nx % kTX != 0 || g.ny % kTY != 0 || w.tiling.zc > (uint32_t)kResidentZ || w.blocks > kPcgThreads) return;
+    int coop = 0, cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, c->device) != hipSuccess || !coop) return;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess) return;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&pcg_resident_kernel<kResidentZ>),
+                                                     kPcgThreads, 0) != hipSuccess)
+        return;
+    if ((uint64_t)per_cu * (uint64_t)cus < w.blocks) return;
+    w.resident_sync = reinterpret_cast<uint32_t*>(device_alloc<float>(c, 2));
+    fill_zero(c, reinterpret_cast<float*>(w.resident_sync), 2);
+    w.resident = true;
+}
+
 // Solves A x = b for the given right-hand-side kind. Returns the solve info; x = (xh, xl).
 ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], const float* const extra[3], float* xh,
                              float* xl, const float* x0h, const float* x0l, double tol, uint32_t max_it) {
@@ -636,12 +715,14 @@ ds_status ds_profile_read(ds_context* c, ds_kernel_timing* out, uint32_t capacit
         double total[kProfiledKernels]{};
         uint64_t samples[kProfiledKernels]{};
         for (const Profiler::Rec& r : prof.recs) {
+            if (r.units == 0) continue;   // a resident launch queued after convergence
             float ms = 0.0f;
             DS_HIP(hipEventElapsedTime(&ms, r.a, r.b));
             total[r.kernel] += ms;
-            samples[r.kernel]++;
+            samples[r.kernel] += r.units;
         }
-        const char* names[kProfiledKernels]{"pcg_direction_kernel", "pcg_update_kernel", "pcg_fused_kernel"};
+        // the resident launch is reported per iteration: launches and samples count iterations
+        const char* names[kProfiledKernels]{"pcg_direction_kernel", "pcg_update_kernel", "pcg_fused_kernel", "pcg_resident_kernel"};
         uint32_t n = 0;
         for (int k = 0; k < kProfiledKernels; ++k) {   // kernels that ran
             if (prof.launches[k] == 0 || n >= capacity) continue;

@@ -206,7 +206,8 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fused_kernel(Geo g, uint32_t 
         double b64 = 0.0, a64;
         if (Mode == kFusedNext) {
             b64 = gamma / gamma_prev;
-            a64 = gamma / (delta + b64 * (red[3] + red[4]) + b64 * b64 * red[5]);
+            // explicit FMAs: the rounding does not depend on the compiler contracting products
+            a64 = gamma / __fma_rn(b64 * b64, red[5], __fma_rn(b64, red[3] + red[4], delta));
         } else {
             a64 = gamma / delta;
         }
@@ -237,17 +238,17 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fused_kernel(Geo g, uint32_t 
                     v.xd[i] = __fmaf_rn(alpha, pi, P.o.d);
                     v.s_out[i] = si;
                     pz = pi;
-                    su += (double)si * u;
-                    sp += (double)si * pi;
+                    su = __fma_rn((double)si, (double)u, su);
+                    sp = __fma_rn((double)si, (double)pi, sp);
                 }
                 if (!kUpdate && P.c.d != 0u) {   // priming: the existing p_{i-1}, s_{i-1}
                     pz = P.o.p;
-                    su += (double)P.o.s * u;
-                    sp += (double)P.o.s * P.o.p;
+                    su = __fma_rn((double)P.o.s, (double)u, su);
+                    sp = __fma_rn((double)P.o.s, (double)P.o.p, sp);
                 }
                 if (kUpdate) v.r_out[i] = P.c.d != 0u ? rn : 0.0f;
-                rho += (double)rn * rn;
-                gam += (double)rn * u;
+                rho = __fma_rn((double)rn, (double)rn, rho);
+                gam = __fma_rn((double)rn, (double)u, gam);
             }
             return u;
         };
@@ -278,12 +279,14 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fused_kernel(Geo g, uint32_t 
                 const uint32_t i = cidx(g, x, y, z);
                 float wv = 0.0f;
                 if (d != 0u)
-                    wv = (float)d * centre -
-                         (((tile[buf][ly + 1][lx] + tile[buf][ly + 1][lx + 2]) + (tile[buf][ly][lx + 1] + tile[buf][ly + 2][lx + 1])) +
-                          below + next);
+                    wv = __fmaf_rn((float)d, centre,
+                                   -((((tile[buf][ly + 1][lx] + tile[buf][ly + 1][lx + 2]) +
+                                       (tile[buf][ly][lx + 1] + tile[buf][ly + 2][lx + 1])) +
+                                      below) +
+                                     next));
                 v.w_out[i] = wv;
-                del += (double)wv * centre;
-                wp += (double)wv * pc;
+                del = __fma_rn((double)wv, (double)centre, del);
+                wp = __fma_rn((double)wv, (double)pc, wp);
             }
             const uint8_t dn = S.c.d;
             load_plane(z + 1 + kFusedDepth, S);

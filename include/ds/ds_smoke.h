@@ -83,7 +83,13 @@ enum { DS_PROJECTION_PCG = 0, DS_PROJECTION_RBGS = 1 };
 /* PCG recurrence: single-reduction (Chronopoulos-Gear: one fused kernel and one global reduction
  * per iteration; the default) or the two-reduction textbook form (two kernels per iteration).
  * Both iterate the same Jacobi-preconditioned CG; they differ only in rounding. */
-enum { DS_CG_SINGLE_REDUCTION = 0, DS_CG_TWO_REDUCTION = 1 };
+/* DS_CG_SINGLE_REDUCTION: one FP64 reduction and one fused kernel launch per iteration (default).
+ * DS_CG_SINGLE_REDUCTION_RESIDENT: the same recurrence as one cooperative launch per
+ * residual-replacement interval with the vectors held on chip, where the grid allows it (nx a
+ * multiple of 64, ny of 8, at most 8 planes per block, every block co-resident), else as the
+ * default; faster on small grids (64^3), slower at 128^3. DS_CG_TWO_REDUCTION: the classic
+ * two-kernel form. */
+enum { DS_CG_SINGLE_REDUCTION = 0, DS_CG_TWO_REDUCTION = 1, DS_CG_SINGLE_REDUCTION_RESIDENT = 2 };
 
 /* Initial guess of the forward and adjoint pressure solves. */
 enum { DS_WARM_START_NONE = 0, DS_WARM_START_PREVIOUS = 1, DS_WARM_START_CNN = 2 };
