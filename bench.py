@@ -4,7 +4,8 @@
 Workload (one bench step): the 128^3 smoke plume, 64 forward steps, then the VJP of
 0.5 ||rho_64 - target||^2 with respect to the initial state by a reverse sweep with a checkpoint
 every 8 steps (recompute in between). Each warm-start mode of the pressure solves (x0 = 0, the
-previous solve's solution, the warm-start network) runs in turn; value = simulated fwd+VJP steps
+previous solve's solution, the linear extrapolation of the previous two solutions, the warm-start
+network) runs in turn; value = simulated fwd+VJP steps
 per second of the fastest, over the timed bench steps, inputs resident in HBM. Also reported: PCG
 iterations per solve (forward, recompute, adjoint) for every mode, the HBM roofline of the
 dominant kernel (HIP-event timing of sampled launches inside the timed region), and the FP64 CPU
@@ -110,8 +111,9 @@ def main():
     ap.add_argument("--profile-every", type=int, default=8)
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--warm-start", choices=["all", "none", "previous", "cnn"], default="all",
-                    help="initial guess of each pressure solve: zero, the previous solve's solution, or the warm-start "
+    ap.add_argument("--warm-start", choices=["all", "none", "previous", "extrapolate", "cnn"], default="all",
+                    help="initial guess of each pressure solve: zero, the previous solve's solution, the linear "
+                         "extrapolation of the previous two, or the warm-start "
                          "network; all = each in turn (the network when its weights exist), value = the fastest")
     ap.add_argument("--weights", default=os.path.join(ROOT, "diff-solver_amd", "data", "warm_start_plume.npz"),
                     help="warm-start network parameters (for --warm-start cnn)")
@@ -137,13 +139,14 @@ def main():
     variant = {"single": abi.CG_SINGLE_REDUCTION, "two": abi.CG_TWO_REDUCTION,
                "resident": abi.CG_SINGLE_REDUCTION_RESIDENT}[args.cg_variant]
     if args.warm_start == "all":
-        modes = ["none", "previous"] + (["cnn"] if os.path.exists(args.weights) else [])
+        modes = ["none", "previous", "extrapolate"] + (["cnn"] if os.path.exists(args.weights) else [])
     else:
         modes = [args.warm_start]
 
     def run_mode(mode):
         """Warm-up, then the timed bench steps of one warm-start mode in a fresh context."""
-        warm = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS, "cnn": abi.WARM_START_CNN}[mode]
+        warm = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS, "extrapolate": abi.WARM_START_EXTRAPOLATE,
+                "cnn": abi.WARM_START_CNN}[mode]
         ctx = Context(sc.config(tolerance=1e-6, warm_start=warm, cg_variant=variant), abi.MODE_DIFFERENTIABLE)
         if mode == "cnn":
             from ds_smoke import warm_start
@@ -203,7 +206,7 @@ def main():
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
                     "algorithmic_bytes_per_launch": d["algorithmic_bytes"], "avg_launch_us": round(d["avg_us"], 2)}
-    label = {"none": "plain", "previous": "previous", "cnn": "cnn"}
+    label = {"none": "plain", "previous": "previous", "extrapolate": "extrapolate", "cnn": "cnn"}
     result = {
         "metric": METRIC,
         "value": round(r["value"], 4),

@@ -127,7 +127,12 @@ struct Collider {
 };
 
 enum class ProjectionSolver : std::uint32_t { pcg = DS_PROJECTION_PCG, rbgs = DS_PROJECTION_RBGS };
-enum class WarmStart : std::uint32_t { none = DS_WARM_START_NONE, previous = DS_WARM_START_PREVIOUS, cnn = DS_WARM_START_CNN };
+enum class WarmStart : std::uint32_t {
+    none = DS_WARM_START_NONE,
+    previous = DS_WARM_START_PREVIOUS,
+    cnn = DS_WARM_START_CNN,
+    extrapolate = DS_WARM_START_EXTRAPOLATE
+};
 enum class CgVariant : std::uint32_t {
     single_reduction = DS_CG_SINGLE_REDUCTION,
     two_reduction = DS_CG_TWO_REDUCTION,

@@ -191,6 +191,10 @@ struct ds_context {
     float* pressure_hi = nullptr;     // solution of the last forward solve (hi + lo)
     float* pressure_lo = nullptr;
     bool pressure_valid = false;
+    // DS_WARM_START_EXTRAPOLATE: the solve before the last one, and the extrapolated guess
+    float* pressure_prev = nullptr;
+    bool pressure_prev_valid = false;
+    float* guess = nullptr;
 
     // adjoint scratch (device, differentiable mode)
     float* sourced_adj[2]{};          // adjoints of rho_s, T_s

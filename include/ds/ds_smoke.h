@@ -92,7 +92,12 @@ enum { DS_PROJECTION_PCG = 0, DS_PROJECTION_RBGS = 1 };
 enum { DS_CG_SINGLE_REDUCTION = 0, DS_CG_TWO_REDUCTION = 1, DS_CG_SINGLE_REDUCTION_RESIDENT = 2 };
 
 /* Initial guess of the forward and adjoint pressure solves. */
-enum { DS_WARM_START_NONE = 0, DS_WARM_START_PREVIOUS = 1, DS_WARM_START_CNN = 2 };
+/* DS_WARM_START_EXTRAPOLATE: each forward pressure solve of a sweep starts from 2 p_{n-1} - p_{n-2}
+ * (the previous solution while only one exists); adjoint solves start from the previous adjoint
+ * solution, as under DS_WARM_START_PREVIOUS. Like every given guess it is dropped (x0 = 0) when
+ * its residual exceeds ||b||. A solve started from the step cache's pressure_guess restarts the
+ * sequence, and so does every trajectory entry point. */
+enum { DS_WARM_START_NONE = 0, DS_WARM_START_PREVIOUS = 1, DS_WARM_START_CNN = 2, DS_WARM_START_EXTRAPOLATE = 3 };
 
 /* smoke::Configuration (data.ixx:72-83) plus the solver fields this build adds. */
 typedef struct ds_config {
