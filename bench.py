@@ -111,9 +111,9 @@ def main():
     ap.add_argument("--profile-every", type=int, default=8)
     ap.add_argument("--cpu-sample-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--warm-start", choices=["all", "none", "previous", "extrapolate", "cnn"], default="all",
+    ap.add_argument("--warm-start", choices=["all", "none", "previous", "extrapolate", "projection", "cnn"], default="all",
                     help="initial guess of each pressure solve: zero, the previous solve's solution, the linear "
-                         "extrapolation of the previous two, or the warm-start "
+                         "extrapolation of the previous two, the projection onto the previous eight, or the warm-start "
                          "network; all = each in turn (the network when its weights exist), value = the fastest")
     ap.add_argument("--weights", default=os.path.join(ROOT, "diff-solver_amd", "data", "warm_start_plume.npz"),
                     help="warm-start network parameters (for --warm-start cnn)")
@@ -139,14 +139,14 @@ def main():
     variant = {"single": abi.CG_SINGLE_REDUCTION, "two": abi.CG_TWO_REDUCTION,
                "resident": abi.CG_SINGLE_REDUCTION_RESIDENT}[args.cg_variant]
     if args.warm_start == "all":
-        modes = ["none", "previous", "extrapolate"] + (["cnn"] if os.path.exists(args.weights) else [])
+        modes = ["none", "previous", "extrapolate", "projection"] + (["cnn"] if os.path.exists(args.weights) else [])
     else:
         modes = [args.warm_start]
 
     def run_mode(mode):
         """Warm-up, then the timed bench steps of one warm-start mode in a fresh context."""
         warm = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS, "extrapolate": abi.WARM_START_EXTRAPOLATE,
-                "cnn": abi.WARM_START_CNN}[mode]
+                "projection": abi.WARM_START_PROJECTION, "cnn": abi.WARM_START_CNN}[mode]
         ctx = Context(sc.config(tolerance=1e-6, warm_start=warm, cg_variant=variant), abi.MODE_DIFFERENTIABLE)
         if mode == "cnn":
             from ds_smoke import warm_start
@@ -206,7 +206,7 @@ def main():
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
                     "algorithmic_bytes_per_launch": d["algorithmic_bytes"], "avg_launch_us": round(d["avg_us"], 2)}
-    label = {"none": "plain", "previous": "previous", "extrapolate": "extrapolate", "cnn": "cnn"}
+    label = {"none": "plain", "previous": "previous", "extrapolate": "extrapolate", "projection": "projection", "cnn": "cnn"}
     result = {
         "metric": METRIC,
         "value": round(r["value"], 4),

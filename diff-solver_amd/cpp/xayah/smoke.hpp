@@ -131,7 +131,8 @@ enum class WarmStart : std::uint32_t {
     none = DS_WARM_START_NONE,
     previous = DS_WARM_START_PREVIOUS,
     cnn = DS_WARM_START_CNN,
-    extrapolate = DS_WARM_START_EXTRAPOLATE
+    extrapolate = DS_WARM_START_EXTRAPOLATE,
+    projection = DS_WARM_START_PROJECTION
 };
 enum class CgVariant : std::uint32_t {
     single_reduction = DS_CG_SINGLE_REDUCTION,

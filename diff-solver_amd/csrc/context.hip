@@ -94,7 +94,7 @@ void validate(const ds_config& cfg, uint32_t mode) {
             if (!(col.size[a] > 0.0F)) bad("collider size must be positive");
     }
     if (cfg.projection_solver > DS_PROJECTION_RBGS) bad("projection_solver out of range");
-    if (cfg.warm_start > DS_WARM_START_EXTRAPOLATE) bad("warm_start out of range");
+    if (cfg.warm_start > DS_WARM_START_PROJECTION) bad("warm_start out of range");
     if (cfg.warm_start == DS_WARM_START_CNN && (cfg.resolution[0] % 64 || cfg.resolution[1] % 16 || cfg.resolution[2] % 4))
         throw Failure{DS_ERROR_UNSUPPORTED, "the warm-start network needs nx % 64 == 0, ny % 16 == 0 and nz % 4 == 0"};
     if (!(cfg.cg_tolerance >= 0.0) || !std::isfinite(cfg.cg_tolerance)) bad("cg_tolerance must be >= 0 and finite");
@@ -395,6 +395,11 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
         if (cfg.warm_start == DS_WARM_START_EXTRAPOLATE) {
             c->pressure_prev = device_alloc<float>(c, n);
             c->guess = device_alloc<float>(c, n);
+        }
+        if (cfg.warm_start == DS_WARM_START_PROJECTION) {
+            c->guess = device_alloc<float>(c, n);
+            window_allocate(c, c->window[0]);
+            if (mode == DS_MODE_DIFFERENTIABLE) window_allocate(c, c->window[1]);
         }
         c->pressure_lo = device_alloc<float>(c, n);
         // PCG work vectors

@@ -126,6 +126,21 @@ struct PlaneTiling {
 
 struct UNet;   // warm-start network (unet.hip)
 
+// DS_WARM_START_PROJECTION (window.hip): the last kWindow solutions of one solve sequence, their
+// images under A and their Gram matrix, as a ring.
+constexpr int kWindow = 8;
+struct SolveWindow {
+    float* x[kWindow]{};
+    float* ax[kWindow]{};
+    double* gram = nullptr;       // kWindow x kWindow, <x_i, A x_j>
+    double* partials = nullptr;   // kWindow x window blocks
+    float* coef = nullptr;        // kWindow
+    uint32_t count = 0, head = 0; // live slots; the next slot to write
+};
+struct WinOrder {
+    int slot[kWindow];            // live slots from newest to oldest
+};
+
 struct PcgWork {
     PlaneTiling tiling{1, 1, 4};
     float* b = nullptr;          // right-hand side (zero off the unknowns)
@@ -195,6 +210,7 @@ struct ds_context {
     float* pressure_prev = nullptr;
     bool pressure_prev_valid = false;
     float* guess = nullptr;
+    ds::SolveWindow window[2];        // DS_WARM_START_PROJECTION: forward and adjoint sequences
 
     // adjoint scratch (device, differentiable mode)
     float* sourced_adj[2]{};          // adjoints of rho_s, T_s
@@ -266,6 +282,14 @@ ds_solve_info pcg_solve(ds_context* c, float* x_hi, float* x_lo, const float* x0
 ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], const float* const extra[3], float* xh,
                              float* xl, const float* x0h, const float* x0l, double tol, uint32_t max_it);
 uint32_t pcg_periodic_bits(const ds_context* c);
+// DS_WARM_START_PROJECTION (window.hip): the window of a right-hand-side kind (null when the mode
+// is off or the kind has none), the guess for right-hand side b (null while the window is empty),
+// adding a solution, clearing both windows, allocating one.
+SolveWindow* window_of(ds_context* c, int kind);
+const float* window_guess(ds_context* c, SolveWindow& w, const float* b);
+void window_add(ds_context* c, SolveWindow& w, const float* hi, const float* lo);
+void window_reset(ds_context* c);
+void window_allocate(ds_context* c, SolveWindow& w);
 // Decides whether the single-reduction solve runs resident (full 64 x kTY tiles, at most 8 planes
 // per block, cooperative launch possible with every block co-resident) and allocates its state.
 void pcg_resident_prepare(ds_context* c);

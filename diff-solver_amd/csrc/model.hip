@@ -533,12 +533,12 @@ ds_status ds_simulate(ds_context* c, const ds_state* initial, const ds_control* 
     const ds_status st = guard([&] {
         check_state(initial, "ds_simulate: initial state has a null field");
         check_state(final_state, "ds_simulate: final state has a null field");
-        c->pressure_prev_valid = false;   // an extrapolated guess spans solves of one sweep only
+        c->pressure_prev_valid = false;   // an extrapolated or projected guess spans solves of one sweep only
+        window_reset(c);
         AbiModel m;
         std::function<const ds_control&(uint32_t)> ctrl = [&](uint32_t s) -> const ds_control& {
             return controls[control_count == 1 ? 0 : s];
         };
-        c->pressure_prev_valid = false;   // an extrapolated guess spans solves of one sweep only
         auto t = xayah::solver::simulate_checkpointed<AbiModel, ACtx, AState, ds_control, ds_parameters>(
             m, x, borrow(*initial), ctrl, *params, steps, steps + 1u, false);
         AState out = borrow(*final_state);
@@ -589,6 +589,8 @@ ds_status trajectory_gradient(const char* who, ds_context* c, const ds_state* in
         std::function<const ds_control&(uint32_t)> ctrl = [&](uint32_t s) -> const ds_control& {
             return controls[control_count == 1 ? 0 : s];
         };
+        c->pressure_prev_valid = false;   // an extrapolated or projected guess spans solves of one sweep only
+        window_reset(c);
         auto t = xayah::solver::simulate_checkpointed<AbiModel, ACtx, AState, ds_control, ds_parameters>(
             m, x, borrow(*initial), ctrl, *params, steps, ti->checkpoint_interval, true);
         // terminal adjoint (and loss)

@@ -617,7 +617,16 @@ ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], c
     }
     max_it = std::min(max_it, c->max_iterations);   // the per-iteration history is sized for this cap
     bool network_guess = false;
-    if (x0h == nullptr && c->config.warm_start == DS_WARM_START_CNN) {
+    // a solve with a given guess (the recompute pass's taped pressure) is not part of the sequence
+    SolveWindow* win = x0h == nullptr ? window_of(c, kind) : nullptr;
+    if (win && win->count > 0) {
+        // the projection onto the previous solutions needs b: set-up without a guess, then again
+        // from the projected guess with b given
+        launch_setup(c, kind, in, nullptr, nullptr, false, xh, xl);
+        x0h = window_guess(c, *win, c->pcg.b);
+        x0l = nullptr;
+        kind = kRhsGiven;
+    } else if (x0h == nullptr && c->config.warm_start == DS_WARM_START_CNN) {
         // the network's guess: b first (set-up without a guess), then the set-up again from the guess
         launch_setup(c, kind, in, nullptr, nullptr, false, xh, xl);
         x0h = unet_infer(c, c->pcg.b);
@@ -647,6 +656,7 @@ ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], c
                                                                   : pcg_iterate(c, xh, xl, tol, max_it, warm, rejected);
     info.initial_residual = warm && !rejected ? std::sqrt(std::max(0.0, rr0)) : info.rhs_norm;
     (void)bb;
+    if (win && info.converged) window_add(c, *win, xh, xl);
     return info;
 }
 

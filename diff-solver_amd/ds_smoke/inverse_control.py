@@ -27,7 +27,7 @@ def run(grid: int = 128, iterations: int = 200, steps: int = 64, k: int = 8, lr:
     from . import abi, scenes
     from .solver import Context
 
-    modes = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS, "extrapolate": abi.WARM_START_EXTRAPOLATE,
+    modes = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS, "extrapolate": abi.WARM_START_EXTRAPOLATE, "projection": abi.WARM_START_PROJECTION,
              "cnn": abi.WARM_START_CNN}
     sc = scenes.plume(grid, target_seed=target_seed)
     ctx = Context(sc.config(tolerance=tolerance, warm_start=modes[warm_start]), abi.MODE_DIFFERENTIABLE, device)
@@ -84,7 +84,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-2)
-    ap.add_argument("--warm-start", choices=["none", "previous", "extrapolate", "cnn"], default="none")
+    ap.add_argument("--warm-start", choices=["none", "previous", "extrapolate", "projection", "cnn"], default="none")
     ap.add_argument("--weights", default=None, help="warm-start network (a file written by ds_smoke.train_warm_start)")
     ap.add_argument("--report", type=int, default=10)
     ap.add_argument("--out", default=None)

@@ -31,7 +31,7 @@ def run(grid=256, solids=8, steps=32, k=8, modes=("none", "previous", "cnn"), we
     from .solver import Context
 
     sc = scenes.plume(grid, solids=solids)
-    warm = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS, "extrapolate": abi.WARM_START_EXTRAPOLATE,
+    warm = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS, "extrapolate": abi.WARM_START_EXTRAPOLATE, "projection": abi.WARM_START_PROJECTION,
             "cnn": abi.WARM_START_CNN}
     s0h, kh = scenes.initial_state(sc), scenes.control(sc)
     results, grads = {}, {}

@@ -97,7 +97,15 @@ enum { DS_CG_SINGLE_REDUCTION = 0, DS_CG_TWO_REDUCTION = 1, DS_CG_SINGLE_REDUCTI
  * solution, as under DS_WARM_START_PREVIOUS. Like every given guess it is dropped (x0 = 0) when
  * its residual exceeds ||b||. A solve started from the step cache's pressure_guess restarts the
  * sequence, and so does every trajectory entry point. */
-enum { DS_WARM_START_NONE = 0, DS_WARM_START_PREVIOUS = 1, DS_WARM_START_CNN = 2, DS_WARM_START_EXTRAPOLATE = 3 };
+/* DS_WARM_START_PROJECTION: each forward and each adjoint solve starts from the A-norm-best
+ * combination of the last 8 solutions of its sequence (Galerkin projection onto their span). */
+enum {
+    DS_WARM_START_NONE = 0,
+    DS_WARM_START_PREVIOUS = 1,
+    DS_WARM_START_CNN = 2,
+    DS_WARM_START_EXTRAPOLATE = 3,
+    DS_WARM_START_PROJECTION = 4
+};
 
 /* smoke::Configuration (data.ixx:72-83) plus the solver fields this build adds. */
 typedef struct ds_config {
