@@ -393,7 +393,7 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
         }
         c->pressure_hi = device_alloc<float>(c, n);
         if (cfg.warm_start == DS_WARM_START_EXTRAPOLATE) {
-            c->pressure_prev = device_alloc<float>(c, n);
+            c->pressure_prev = device_alloc<float>(c, 2 * (size_t)n);
             c->guess = device_alloc<float>(c, n);
         }
         if (cfg.warm_start == DS_WARM_START_PROJECTION) {

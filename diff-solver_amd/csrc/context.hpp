@@ -206,9 +206,10 @@ struct ds_context {
     float* pressure_hi = nullptr;     // solution of the last forward solve (hi + lo)
     float* pressure_lo = nullptr;
     bool pressure_valid = false;
-    // DS_WARM_START_EXTRAPOLATE: the solve before the last one, and the extrapolated guess
+    // DS_WARM_START_EXTRAPOLATE: the two solutions before the last one (p_{n-2} at [0, n), p_{n-3}
+    // at [n, 2n)), how many of them hold a solution of the current sweep, and the extrapolated guess
     float* pressure_prev = nullptr;
-    bool pressure_prev_valid = false;
+    int pressure_prev_count = 0;
     float* guess = nullptr;
     ds::SolveWindow window[2];        // DS_WARM_START_PROJECTION: forward and adjoint sequences
 

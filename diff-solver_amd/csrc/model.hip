@@ -80,37 +80,36 @@ void zero_buffers(ds_context* c, std::initializer_list<std::pair<float*, uint32_
     check_launch("zero_list_kernel");
 }
 
-// guess = 2 (hi + lo) - prev, then prev = hi + lo: the linear extrapolation of the last two
-// solutions, and the sequence moved on by one.
-__global__ __launch_bounds__(256) void extrapolate_kernel(uint32_t n, const float* __restrict__ hi, const float* __restrict__ lo,
+// The extrapolated guess from the last solution x = hi + lo and the earlier ones in prev (p1 at
+// [0, n), p2 at [n, 2n)): order 1, 2 x - p1; order 2, 3 (x - p1) + p2 (the polynomial through the
+// last two or three solutions, at the next step); order 0 only records x. The history moves on by
+// one: p2 = p1, p1 = x.
+__global__ __launch_bounds__(256) void extrapolate_kernel(uint32_t n, int order, const float* __restrict__ hi, const float* __restrict__ lo,
                                                           float* __restrict__ prev, float* __restrict__ guess) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const float x = hi[i] + lo[i];
-        guess[i] = 2.0f * x - prev[i];
+        const float p1 = prev[i];
+        if (order == 2) guess[i] = 3.0f * (x - p1) + prev[n + i];
+        else if (order == 1) guess[i] = 2.0f * x - p1;
+        prev[n + i] = p1;
         prev[i] = x;
     }
 }
-__global__ __launch_bounds__(256) void keep_previous_kernel(uint32_t n, const float* __restrict__ hi, const float* __restrict__ lo,
-                                                            float* __restrict__ prev) {
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) prev[i] = hi[i] + lo[i];
-}
 
 // The initial guess of a solve whose last solution is (hi, lo), valid or not, under
-// DS_WARM_START_PREVIOUS (the last solution, in place) or DS_WARM_START_EXTRAPOLATE (2 x_{n-1} -
-// x_{n-2} once two exist). Returns the guess pair (null: x0 = 0) and moves the sequence on.
-std::pair<const float*, const float*> sequence_guess(ds_context* c, float* hi, float* lo, bool valid, float* prev, bool& prev_valid) {
+// DS_WARM_START_PREVIOUS (the last solution, in place) or DS_WARM_START_EXTRAPOLATE (the quadratic
+// extrapolation of the last three solutions of the sweep, the linear one of the last two, the last
+// solution). Returns the guess pair (null: x0 = 0) and moves the history on. prev = null: the
+// previous solution only.
+std::pair<const float*, const float*> sequence_guess(ds_context* c, float* hi, float* lo, bool valid, float* prev, int& prev_count) {
     const uint32_t mode = c->config.warm_start;
     if (!valid || (mode != DS_WARM_START_PREVIOUS && mode != DS_WARM_START_EXTRAPOLATE)) return {nullptr, nullptr};
     if (mode == DS_WARM_START_EXTRAPOLATE && prev) {
-        const uint32_t n = c->geo.n;
-        if (prev_valid) {
-            extrapolate_kernel<<<2048, kBlock, 0, c->stream>>>(n, hi, lo, prev, c->guess);
-            check_launch("extrapolate_kernel");
-            return {c->guess, nullptr};
-        }
-        keep_previous_kernel<<<2048, kBlock, 0, c->stream>>>(n, hi, lo, prev);
-        check_launch("keep_previous_kernel");
-        prev_valid = true;
+        const int order = prev_count;
+        extrapolate_kernel<<<2048, kBlock, 0, c->stream>>>(c->geo.n, order, hi, lo, prev, c->guess);
+        check_launch("extrapolate_kernel");
+        prev_count = std::min(prev_count + 1, 2);
+        if (order > 0) return {c->guess, nullptr};
     }
     return {hi, lo};
 }
@@ -156,9 +155,9 @@ ds_solve_info forward_step(ds_context* c, const ds_state* s, const ds_control* k
     if (cache->pressure_guess) {
         x0h = cache->pressure_guess;
         x0l = cache->pressure_guess + n;
-        c->pressure_prev_valid = false;   // a solve out of the step sequence
+        c->pressure_prev_count = 0;   // a solve out of the step sequence
     } else {
-        const auto g = sequence_guess(c, c->pressure_hi, c->pressure_lo, c->pressure_valid, c->pressure_prev, c->pressure_prev_valid);
+        const auto g = sequence_guess(c, c->pressure_hi, c->pressure_lo, c->pressure_valid, c->pressure_prev, c->pressure_prev_count);
         x0h = g.first;
         x0l = g.second;
     }
@@ -211,7 +210,7 @@ ds_solve_info vjp_step(ds_context* c, const ds_parameters* p, const ds_state* st
     // adjoint solves start from the previous adjoint solution under both sequence modes: on the
     // 128^3 plume the extrapolated adjoint guess took more iterations (606 per solve) than the
     // previous solution (566)
-    bool no_prev = false;
+    int no_prev = 0;
     const auto lg = sequence_guess(c, c->lambda_hi, c->lambda_lo, c->lambda_valid, nullptr, no_prev);
     const ds_solve_info si = pcg_solve_kind(c, 2, na, ex, c->lambda_hi, c->lambda_lo, lg.first, lg.second, c->tolerance, c->max_iterations);
     c->lambda_valid = true;
@@ -533,7 +532,7 @@ ds_status ds_simulate(ds_context* c, const ds_state* initial, const ds_control* 
     const ds_status st = guard([&] {
         check_state(initial, "ds_simulate: initial state has a null field");
         check_state(final_state, "ds_simulate: final state has a null field");
-        c->pressure_prev_valid = false;   // an extrapolated or projected guess spans solves of one sweep only
+        c->pressure_prev_count = 0;   // an extrapolated or projected guess spans solves of one sweep only
         window_reset(c);
         AbiModel m;
         std::function<const ds_control&(uint32_t)> ctrl = [&](uint32_t s) -> const ds_control& {
@@ -589,7 +588,7 @@ ds_status trajectory_gradient(const char* who, ds_context* c, const ds_state* in
         std::function<const ds_control&(uint32_t)> ctrl = [&](uint32_t s) -> const ds_control& {
             return controls[control_count == 1 ? 0 : s];
         };
-        c->pressure_prev_valid = false;   // an extrapolated or projected guess spans solves of one sweep only
+        c->pressure_prev_count = 0;   // an extrapolated or projected guess spans solves of one sweep only
         window_reset(c);
         auto t = xayah::solver::simulate_checkpointed<AbiModel, ACtx, AState, ds_control, ds_parameters>(
             m, x, borrow(*initial), ctrl, *params, steps, ti->checkpoint_interval, true);

@@ -689,8 +689,8 @@ ds_status ds_pcg_solve(ds_context* c, const float* b, float* x, const float* x0,
         // solve into the context's pressure pair, then round hi + lo into x
         si = pcg_solve_kind(c, kRhsGiven, nullptr, nullptr, c->pressure_hi, c->pressure_lo, x0, nullptr, tol, mi);
         c->pressure_valid = false;
-        c->pressure_prev_valid = false;
-        c->pressure_prev_valid = false;
+        c->pressure_prev_count = 0;
+        c->pressure_prev_count = 0;
     });
     if (st != DS_OK) return st;
     st = guard([&] {
@@ -775,7 +775,7 @@ ds_status ds_warm_start_infer(ds_context* c, const float* b, float* x) {
         RhsIn in{};
         launch_setup(c, kRhsGiven, in, nullptr, nullptr, false, c->pressure_hi, c->pressure_lo);   // ||b||^2 partials
         c->pressure_valid = false;
-        c->pressure_prev_valid = false;
+        c->pressure_prev_count = 0;
         const float* g = unet_infer(c, c->pcg.b);
         DS_HIP(hipMemcpyAsync(x, g, (size_t)c->geo.n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
         DS_HIP(hipStreamSynchronize(c->stream));

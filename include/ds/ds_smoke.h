@@ -92,11 +92,12 @@ enum { DS_PROJECTION_PCG = 0, DS_PROJECTION_RBGS = 1 };
 enum { DS_CG_SINGLE_REDUCTION = 0, DS_CG_TWO_REDUCTION = 1, DS_CG_SINGLE_REDUCTION_RESIDENT = 2 };
 
 /* Initial guess of the forward and adjoint pressure solves. */
-/* DS_WARM_START_EXTRAPOLATE: each forward pressure solve of a sweep starts from 2 p_{n-1} - p_{n-2}
- * (the previous solution while only one exists); adjoint solves start from the previous adjoint
- * solution, as under DS_WARM_START_PREVIOUS. Like every given guess it is dropped (x0 = 0) when
- * its residual exceeds ||b||. A solve started from the step cache's pressure_guess restarts the
- * sequence, and so does every trajectory entry point. */
+/* DS_WARM_START_EXTRAPOLATE: each forward pressure solve of a sweep starts from the quadratic
+ * extrapolation 3 (p_{n-1} - p_{n-2}) + p_{n-3} of its last three solutions (the linear one,
+ * 2 p_{n-1} - p_{n-2}, or the previous solution while fewer exist); adjoint solves start from the
+ * previous adjoint solution, as under DS_WARM_START_PREVIOUS. Like every given guess it is dropped
+ * (x0 = 0) when its residual exceeds ||b||. A solve started from the step cache's pressure_guess
+ * restarts the sequence, and so does every trajectory entry point. */
 /* DS_WARM_START_PROJECTION: each forward and each adjoint solve starts from the A-norm-best
  * combination of the last 8 solutions of its sequence (Galerkin projection onto their span). */
 enum {
