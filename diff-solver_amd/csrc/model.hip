@@ -101,8 +101,10 @@ __global__ __launch_bounds__(256) void extrapolate_kernel(uint32_t n, int order,
 // extrapolation of the last three solutions of the sweep, the linear one of the last two, the last
 // solution). Returns the guess pair (null: x0 = 0) and moves the history on. prev = null: the
 // previous solution only.
-std::pair<const float*, const float*> sequence_guess(ds_context* c, float* hi, float* lo, bool valid, float* prev, int& prev_count) {
-    const uint32_t mode = c->config.warm_start;
+std::pair<const float*, const float*> sequence_guess(ds_context* c, float* hi, float* lo, bool valid, float* prev, int& prev_count,
+                                                     bool previous_under_network = false) {
+    uint32_t mode = c->config.warm_start;
+    if (previous_under_network && mode == DS_WARM_START_CNN) mode = DS_WARM_START_PREVIOUS;
     if (!valid || (mode != DS_WARM_START_PREVIOUS && mode != DS_WARM_START_EXTRAPOLATE)) return {nullptr, nullptr};
     if (mode == DS_WARM_START_EXTRAPOLATE && prev) {
         const int order = prev_count;
@@ -207,11 +209,12 @@ ds_solve_info vjp_step(ds_context* c, const ds_parameters* p, const ds_state* st
                              c->sourced_adj[0], c->sourced_adj[1], F3{{c->projected_extra[0], c->projected_extra[1], c->projected_extra[2]}});
     const float* na[3]{next_adj->velocity[0], next_adj->velocity[1], next_adj->velocity[2]};
     const float* ex[3]{c->projected_extra[0], c->projected_extra[1], c->projected_extra[2]};
-    // adjoint solves start from the previous adjoint solution under both sequence modes: on the
-    // 128^3 plume the extrapolated adjoint guess took more iterations (606 per solve) than the
-    // previous solution (566)
+    // adjoint solves start from the previous adjoint solution under both sequence modes (on the
+    // 128^3 plume the extrapolated adjoint guess took more iterations, 606 per solve, than the
+    // previous solution, 566) and under the network, which is trained on forward pairs only (its
+    // guess cost 923 iterations per adjoint solve against 804 from x0 = 0)
     int no_prev = 0;
-    const auto lg = sequence_guess(c, c->lambda_hi, c->lambda_lo, c->lambda_valid, nullptr, no_prev);
+    const auto lg = sequence_guess(c, c->lambda_hi, c->lambda_lo, c->lambda_valid, nullptr, no_prev, true);
     const ds_solve_info si = pcg_solve_kind(c, 2, na, ex, c->lambda_hi, c->lambda_lo, lg.first, lg.second, c->tolerance, c->max_iterations);
     c->lambda_valid = true;
     launch_projection_vjp_faces(c, na, ex, c->lambda_hi, c->lambda_lo, F3{{c->raw_adj[0], c->raw_adj[1], c->raw_adj[2]}}, true);

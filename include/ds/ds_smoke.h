@@ -354,8 +354,10 @@ ds_status ds_apply_pressure_matrix(ds_context* c, const float* x, float* y);
 uint64_t ds_warm_start_parameter_count(void);
 /* Loads the network's parameters (PyTorch state_dict order of ds_smoke/warm_start.py's UNet:
  * per layer the weight, then the bias) and the scale of its output. Needs nx % 64 == 0,
- * ny % 16 == 0, nz % 4 == 0. With DS_WARM_START_CNN every solve without an explicit guess starts
- * from the network's guess; the guess is rejected (x0 = 0) when it is worse than 0. */
+ * ny % 16 == 0, nz % 4 == 0. With DS_WARM_START_CNN every forward solve without an explicit
+ * guess starts from the network's guess, scaled to its A-norm-best multiple (rejected, x0 = 0, when
+ * non-finite); the adjoint solves of ds_vjp_step start from the previous adjoint solution, since
+ * the network is trained on forward pairs. */
 ds_status ds_warm_start_load(ds_context* c, const float* params, uint64_t count, double output_scale);
 /* The network's guess x for A x = b (device cell fields; b is 0 off the unknowns). */
 ds_status ds_warm_start_infer(ds_context* c, const float* b, float* x);
