@@ -145,6 +145,30 @@ def test_forward_simulation_matches_oracle():
     sim.close()
 
 
+def test_full_size_steps_match_oracle():
+    """The reference's own configuration (128 x 192 x 128, RBGS 160, confinement 2): the first two
+    steps against the oracle's."""
+    from ds_smoke.example import ForwardSimulation, ForwardSimulationOptions
+
+    opts = ForwardSimulationOptions()
+    steps = 2
+    print(f"ForwardSimulation 128x192x128 RBGS {opts.pressure_iterations}, {steps} steps against the oracle", flush=True)
+    sim = ForwardSimulation(opts)
+    for _ in range(steps):
+        sim.step()
+    got = sim.context.to_host(sim.current_state)
+    ref_state, ref_raw = _oracle_run(opts, steps)
+    for name in ("density", "temperature"):
+        err = rel(got[name], ref_state[name])
+        print(f"  {name}: relative L2 {err:.2e}")
+        assert err < 1e-5, (name, err)
+    for a in range(3):
+        err = rel(got["velocity"][a], ref_state["velocity"][a])
+        print(f"  velocity[{a}]: relative L2 {err:.2e}")
+        assert err < 1e-4, (a, err)
+    sim.close()
+
+
 def test_cpp_program_matches_python_harness(tmp_path):
     from ds_smoke.example import ForwardSimulation
 
