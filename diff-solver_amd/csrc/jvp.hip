@@ -280,9 +280,10 @@ void launch_advect_scalar_jvp(ds_context* c, int nf, int first_field, const floa
 }
 
 // projection_jvp (operators.cu:1015-1018): the tangent pressure solve (warm-started from the
-// previous tangent solve when warm_start is DS_WARM_START_PREVIOUS) and the forward projection.
+// previous tangent solve under every warm start other than DS_WARM_START_NONE: the tangent
+// sequence has no extrapolation, window or network of its own) and the forward projection.
 ds_solve_info project_tangent(ds_context* c, CF3 ut, F3 out) {
-    const bool warm = c->config.warm_start == DS_WARM_START_PREVIOUS && c->tangent_valid;
+    const bool warm = c->config.warm_start != DS_WARM_START_NONE && c->tangent_valid;
     const float* u[3]{ut.p[0], ut.p[1], ut.p[2]};
     const ds_solve_info si = pcg_solve_kind(c, 3, u, nullptr, c->tangent_hi, c->tangent_lo, warm ? c->tangent_hi : nullptr,
                                             warm ? c->tangent_lo : nullptr, c->tolerance, c->max_iterations);
