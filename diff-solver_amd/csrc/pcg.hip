@@ -690,7 +690,6 @@ ds_status ds_pcg_solve(ds_context* c, const float* b, float* x, const float* x0,
         si = pcg_solve_kind(c, kRhsGiven, nullptr, nullptr, c->pressure_hi, c->pressure_lo, x0, nullptr, tol, mi);
         c->pressure_valid = false;
         c->pressure_prev_count = 0;
-        c->pressure_prev_count = 0;
     });
     if (st != DS_OK) return st;
     st = guard([&] {
