@@ -109,7 +109,7 @@ def main():
     ap.add_argument("--sim-steps", type=int, default=64)
     ap.add_argument("--checkpoint", type=int, default=8)
     ap.add_argument("--profile-every", type=int, default=8)
-    ap.add_argument("--cpu-sample-steps", type=int, default=2)
+    ap.add_argument("--cpu-sample-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--warm-start", choices=["all", "none", "previous", "extrapolate", "projection", "cnn"], default="all",
                     help="initial guess of each pressure solve: zero, the previous solve's solution, the linear "
