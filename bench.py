@@ -192,8 +192,10 @@ def main():
             kern[name] = {"launches": launches, "samples": samples, "avg_us": avg_ms * 1e3, "algorithmic_bytes": b,
                           "gbs": b / (avg_ms * 1e-3) / 1e9 if samples else None}
         ctx.close()
+        iters = (info.forward_iterations + info.recompute_iterations + info.adjoint_iterations) / args.sim_steps
         return {"value": args.steps * args.sim_steps / elapsed, "ms_per_step": elapsed / args.steps * 1e3, "loss": loss,
-                "per_solve": per_solve, "kernels": kern, "not_converged": info.not_converged}
+                "per_solve": per_solve, "kernels": kern, "not_converged": info.not_converged,
+                "cells": n, "unknowns": nf, "cg_iterations_per_sim_step": iters}
 
     results = {m: run_mode(m) for m in modes}
     best = max(results, key=lambda m: results[m]["value"])
@@ -206,6 +208,13 @@ def main():
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(d["gbs"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic_from_profiles(dom),
                     "algorithmic_bytes_per_launch": d["algorithmic_bytes"], "avg_launch_us": round(d["avg_us"], 2)}
+    # whole fwd+VJP step at the reference's operator decomposition: 311 B/cell forward, 311 B/cell
+    # recompute, 605 B/cell reverse, 47 B per unknown per CG iteration (the two-kernel textbook PCG)
+    step_s = r["ms_per_step"] * 1e-3 / args.sim_steps
+    step_bytes = r["cells"] * (311 + 311 + 605) + 47 * r["unknowns"] * r["cg_iterations_per_sim_step"]
+    step_gbs = step_bytes / step_s / 1e9
+    step_roofline = {"bound": "hbm", "bytes_per_step": round(step_bytes), "cg_iterations_per_step": round(r["cg_iterations_per_sim_step"], 1),
+                     "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
     label = {"none": "plain", "previous": "previous", "extrapolate": "extrapolate", "projection": "projection", "cnn": "cnn"}
     result = {
         "metric": METRIC,
@@ -228,6 +237,7 @@ def main():
         "steps_per_s_by_warm_start": {label[m]: round(results[m]["value"], 4) for m in modes},
         "loss": r["loss"],
         "roofline": roofline,
+        "step_roofline": step_roofline,
         "kernels": {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in kern.items()},
     }
     if not args.no_cpu_baseline:
