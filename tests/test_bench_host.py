@@ -1,5 +1,6 @@
-"""Host-side behaviour of bench.py that needs no GPU: the hot path runs on one GPU, so a request for
-more exits with status 2 and says so before touching a device."""
+"""Host-side behaviour that needs no GPU: bench.py runs the hot path on one GPU, so a request for
+more exits with status 2 and says so before touching a device; and the solver library loader
+raises, naming the path, when the built HIP library is missing (there is no CPU fallback)."""
 import os
 import subprocess
 import sys
@@ -13,3 +14,13 @@ def test_more_than_one_gpu_is_refused():
     assert out.returncode == 2
     assert "one GPU only" in out.stderr and "--gpus 2" in out.stderr
     assert out.stdout == ""
+
+
+def test_missing_library_fails_loudly():
+    # the product path has no CPU fallback: without the built HIP library the loader raises
+    env = dict(os.environ, DS_SMOKE_LIBRARY=os.path.join(ROOT, "no_such_dir", "libds_smoke.so"),
+               PYTHONPATH=os.path.join(ROOT, "diff-solver_amd"))
+    code = "from ds_smoke.solver import load_library\nload_library()\n"
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300, cwd=ROOT, env=env)
+    assert out.returncode != 0
+    assert "HIP smoke library not found" in out.stderr and "no_such_dir" in out.stderr
