@@ -161,8 +161,10 @@ struct Corner8 {   // the 2x2x2 trilinear stencil: base cell and fractional offs
     float tx, ty, tz;
 };
 
+// Grid coordinates divide by h as the reference does (operators.cu:137-139,167-172): with a cell
+// size that is not a power of two, p * (1/h) can round to the other side of a cell boundary.
 __device__ __forceinline__ Corner8 corners(const Geo& g, float px, float py, float pz, float ox, float oy, float oz) {
-    const float gx = px * g.inv_h - ox, gy = py * g.inv_h - oy, gz = pz * g.inv_h - oz;
+    const float gx = px / g.h - ox, gy = py / g.h - oy, gz = pz / g.h - oz;
     Corner8 c;
     const float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
     c.x0 = (int)fx;
@@ -337,9 +339,9 @@ __device__ __forceinline__ bool solid_at(float px, float py, float pz, const uin
     if (vperiodic(b, 0)) { const float L = g.nx * g.h; px -= floorf(px / L) * L; }
     if (vperiodic(b, 1)) { const float L = g.ny * g.h; py -= floorf(py / L) * L; }
     if (vperiodic(b, 2)) { const float L = g.nz * g.h; pz -= floorf(pz / L) * L; }
-    const int x = max(0, min(g.nx - 1, (int)floorf(px * g.inv_h)));
-    const int y = max(0, min(g.ny - 1, (int)floorf(py * g.inv_h)));
-    const int z = max(0, min(g.nz - 1, (int)floorf(pz * g.inv_h)));
+    const int x = max(0, min(g.nx - 1, (int)floorf(px / g.h)));
+    const int y = max(0, min(g.ny - 1, (int)floorf(py / g.h)));
+    const int z = max(0, min(g.nz - 1, (int)floorf(pz / g.h)));
     return mask[cidx(g, x, y, z)] != 0;
 }
 

@@ -1,0 +1,87 @@
+"""PCG and the step on grids wider than one tile of the plane-marching kernels.
+
+The PCG kernels tile x by 64 cells; a block rebuilds the cells beyond its tile's x rims from its
+neighbours' data, so nx > 64 that is not a multiple of 64 exercises both an interior seam and a
+ragged last tile. 136 x 40 x 24 (tiles of 64, 64 and 8 cells) with spheres that straddle the
+seams at x = 64 and x = 128, closed and periodic in x (operators.cu:852-902 for the periodic
+neighbour rule): ds_pcg_solve and forward_step against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from helpers import collider, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def seam_scene(periodic_x):
+    from ds_smoke import abi, scenes
+
+    h = 1.0 / 64.0
+    sc = scenes.Scene(resolution=(136, 40, 24), cell_size=h, time_step=0.5 * h, name="seam" + ("_px" if periodic_x else ""))
+    sc.colliders = [collider(abi.SHAPE_ELLIPSOID, (64 * h, 20 * h, 12 * h), (6.5 * h, 7.5 * h, 5.5 * h)),
+                    collider(abi.SHAPE_ELLIPSOID, (128.5 * h, 12 * h, 14 * h), (5.2 * h, 4.8 * h, 6.1 * h)),
+                    collider(abi.SHAPE_BOX, (100 * h, 30 * h, 8 * h), (3.1 * h, 2.6 * h, 2.2 * h), velocity=(0.05, 0.0, 0.0))]
+    if periodic_x:
+        sc.velocity_modes = (abi.VELOCITY_PERIODIC, abi.VELOCITY_PERIODIC, 0, 0, 0, 0)
+        sc.pressure_modes = (abi.SCALAR_PERIODIC, abi.SCALAR_PERIODIC, 1, 1, 1, 1)
+        sc.density_modes = (abi.SCALAR_PERIODIC, abi.SCALAR_PERIODIC, 1, 1, 1, 1)
+        sc.temperature_modes = (abi.SCALAR_PERIODIC, abi.SCALAR_PERIODIC, 1, 1, 1, 1)
+    return sc
+
+
+@pytest.fixture(scope="module", params=[False, True], ids=["closed", "periodic_x"])
+def seam(request):
+    import oracle as O
+    from ds_smoke import abi
+    from ds_smoke.solver import Context
+
+    sc = seam_scene(request.param)
+    ctx = Context(sc.config(tolerance=1e-6), abi.MODE_DIFFERENTIABLE)
+    orc = O.Oracle(sc, tolerance=1e-10)
+    return request.param, sc, ctx, orc
+
+
+def test_pcg_across_tiles(seam):
+    periodic, sc, ctx, orc = seam
+    print(f"seam pcg periodic_x={periodic}", flush=True)
+    _, code = ctx.masks()
+    np.testing.assert_array_equal(code, orc.masks()[1])
+    rng = np.random.default_rng(136)
+    b = np.where(code > 0, rng.standard_normal(code.shape), 0.0).astype(np.float32)
+    x, info, st = ctx.pcg_solve(ctx.to_device(b))
+    assert st == 0 and info.converged
+    ref, _ = orc.pcg(b.astype(np.float64), tol=1e-12)
+    err = rel(ctx.to_host(x), ref)
+    # the matrix-vector product itself, cell by cell, against the oracle's
+    xs = rng.standard_normal(code.shape).astype(np.float32) * (code > 0)
+    ax = ctx.to_host(ctx.apply_matrix(ctx.to_device(xs))).astype(np.float64)
+    ax_ref = orc.apply_matrix(xs.astype(np.float64))
+    print(f"  {info.iterations} iterations; solution {err:.2e}; A x max |diff| {np.abs(ax - ax_ref).max():.2e}")
+    assert err < 5e-6, err
+    np.testing.assert_allclose(ax, ax_ref, rtol=0, atol=1e-5)
+
+
+def test_step_across_tiles(seam):
+    from ds_smoke.scenes import cell_shape, face_shape
+
+    periodic, sc, ctx, orc = seam
+    n = sc.resolution
+    rng = np.random.default_rng(7)
+    s = {"density": np.abs(rng.standard_normal(cell_shape(n))).astype(np.float32),
+         "temperature": rng.standard_normal(cell_shape(n)).astype(np.float32),
+         "velocity": tuple((0.4 * rng.standard_normal(face_shape(n, a))).astype(np.float32) for a in range(3))}
+    z = np.zeros(cell_shape(n), dtype=np.float32)
+    k = {"density_source": z, "temperature_source": z, "external_acceleration": (z, z, z)}
+    params = ctx.allocate_parameters(sc.params)
+    nxt, cache, info = ctx.forward_step(ctx.state_from_host(s), ctx.control_from_host(k), params)
+    assert info.converged
+    ref, ref_cache, _ = orc.forward_step(s, k, sc.params)
+    got = ctx.to_host(nxt)
+    errs = [rel(got["density"], ref["density"])] + [rel(got["velocity"][a], ref["velocity"][a]) for a in range(3)]
+    gc = ctx.to_host(cache)
+    p = gc["pressure"][0].astype(np.float64) + gc["pressure"][1]
+    ep = rel(p, ref_cache["pressure"])
+    print(f"  step periodic_x={periodic}: state {max(errs):.2e}, pressure {ep:.2e}")
+    assert max(errs) < 1e-5, errs
+    assert ep < 1e-5, ep

@@ -127,6 +127,9 @@ def test_loss_gradient_matches_oracle(k):
     ref = orc.loss_gradient(s0, kk, sc.params, 8, k, sc.target)
     assert abs(loss - ref["loss"]) / ref["loss"] < 1e-5
     g = ctx.to_host(grad)
+    errs = [rel(g["velocity"][a], ref["grad"]["velocity"][a]) for a in range(3)]
+    e_par = rel(ctx.to_host(pa)[:3], ref["params"][:3])
+    print(f"k = {k}: loss {abs(loss - ref['loss']) / ref['loss']:.2e}, velocity gradient {max(errs):.2e}, parameters {e_par:.2e}")
     for a in range(3):
-        assert rel(g["velocity"][a], ref["grad"]["velocity"][a]) < 2e-3, (a, rel(g["velocity"][a], ref["grad"]["velocity"][a]))
+        assert errs[a] < 2e-3, (a, errs[a])
     np.testing.assert_allclose(ctx.to_host(pa)[:3], ref["params"][:3], rtol=2e-3)
