@@ -80,101 +80,116 @@ concept JvpModel = ForwardModel<Model, Context, State, Control, Parameters> &&
                        { m.jvp_step(s, k, p, n, cache, st, kt, pt, nt, c) } -> std::same_as<void>;
                    };
 
-// solver::simulate, solver.ixx:53-81.
+namespace detail {
+
+// The step caches of a stored trajectory, in any order: the taped cache of a step under
+// store_all; under recompute_step_cache the step is replayed from its stored input state into
+// one scratch state and one scratch cache, which the returned reference then points at (valid
+// until the next call).
+template <typename Model, typename Context, typename State, typename StepCache, typename Control, typename Parameters>
+class CacheReplay {
+public:
+    CacheReplay(const Model& model, Context& context, const Trajectory<State, StepCache>& trajectory, std::span<const Control> controls,
+                const Parameters& parameters)
+        : model_(model), context_(context), trajectory_(trajectory), controls_(controls), parameters_(parameters) {
+        if (trajectory.tape_mode == TapeMode::recompute_step_cache && !controls.empty()) {
+            scratch_state_.emplace(model.allocate_state(context));
+            scratch_cache_.emplace(model.allocate_step_cache(context));
+        }
+    }
+    const StepCache& operator()(std::size_t step) {
+        if (!scratch_cache_) return trajectory_.step_caches[step];
+        model_.forward_step(trajectory_.states[step], controls_[step], parameters_, *scratch_state_, *scratch_cache_, context_);
+        return *scratch_cache_;
+    }
+
+private:
+    const Model& model_;
+    Context& context_;
+    const Trajectory<State, StepCache>& trajectory_;
+    std::span<const Control> controls_;
+    const Parameters& parameters_;
+    std::optional<State> scratch_state_;
+    std::optional<StepCache> scratch_cache_;
+};
+
+template <typename T, typename Make>
+std::vector<T> make_n(std::size_t n, Make&& make) {
+    std::vector<T> v;
+    v.reserve(n);
+    for (std::size_t i = 0; i < n; ++i) v.push_back(make());
+    return v;
+}
+
+}  // namespace detail
+
+// solver::simulate (solver.ixx:53-81): the n + 1 states of the trajectory (element 0 a copy of the
+// initial state) and, under store_all, the n step caches. Every buffer is allocated before the
+// first step; under recompute_step_cache one cache serves every step and is dropped at the end.
 template <typename Model, typename Context, typename State, typename Control, typename Parameters>
     requires ForwardModel<Model, Context, State, Control, Parameters>
 [[nodiscard]] auto simulate(const Model& model, Context& context, const State& initial_state, std::span<const Control> controls,
                             const Parameters& parameters, TapeMode tape_mode)
     -> Trajectory<State, decltype(model.allocate_step_cache(context))> {
     using Cache = decltype(model.allocate_step_cache(context));
-    Trajectory<State, Cache> t{tape_mode, {}, {}};
-    t.states.reserve(controls.size() + 1);
-    State first = model.allocate_state(context);
-    model.copy_state(initial_state, first, context);
-    t.states.push_back(std::move(first));
-    std::optional<Cache> reusable;
-    if (tape_mode == TapeMode::recompute_step_cache && !controls.empty()) reusable.emplace(model.allocate_step_cache(context));
-    for (const Control& control : controls) {
-        State next = model.allocate_state(context);
-        if (tape_mode == TapeMode::store_all) {
-            Cache cache = model.allocate_step_cache(context);
-            model.forward_step(t.states.back(), control, parameters, next, cache, context);
-            t.step_caches.push_back(std::move(cache));
-        } else {
-            model.forward_step(t.states.back(), control, parameters, next, *reusable, context);
-        }
-        t.states.push_back(std::move(next));
-    }
-    return t;
+    const std::size_t n = controls.size();
+    const bool taped = tape_mode == TapeMode::store_all;
+    std::vector<State> states = detail::make_n<State>(n + 1, [&] { return model.allocate_state(context); });
+    std::vector<Cache> caches = detail::make_n<Cache>(taped ? n : std::min<std::size_t>(n, 1),
+                                                      [&] { return model.allocate_step_cache(context); });
+    model.copy_state(initial_state, states.front(), context);
+    for (std::size_t step = 0; step < n; ++step)
+        model.forward_step(states[step], controls[step], parameters, states[step + 1], caches[taped ? step : 0], context);
+    if (!taped) caches.clear();
+    return {tape_mode, std::move(states), std::move(caches)};
 }
 
-// solver::vjp, solver.ixx:115-150.
+// solver::vjp (solver.ixx:115-150): trajectory_adjoint holds dL/d(state i) for i = 0..n. The
+// reverse sweep carries the adjoint of state step + 1 in one buffer and builds the adjoint of
+// state step in a second one: seeded with the loss adjoint of that state, then vjp_step adds the
+// step's pullback (vjp_step accumulates into its output). The two buffers swap roles each step.
+// Control and parameter adjoints accumulate over the steps as vjp_step adds into them.
 template <typename Model, typename Context, typename State, typename StepCache, typename Control, typename Parameters, typename StateAdjoint>
     requires VjpModel<Model, Context, State, Control, Parameters>
 [[nodiscard]] auto vjp(const Model& model, Context& context, const Trajectory<State, StepCache>& trajectory, std::span<const Control> controls,
                        const Parameters& parameters, std::span<const StateAdjoint> trajectory_adjoint)
     -> GradientResult<StateAdjoint, decltype(model.allocate_control_adjoint(context)), decltype(model.allocate_parameter_adjoint(context))> {
-    GradientResult<StateAdjoint, decltype(model.allocate_control_adjoint(context)), decltype(model.allocate_parameter_adjoint(context))> r{
-        model.allocate_state_adjoint(context), {}, model.allocate_parameter_adjoint(context)};
-    r.controls.reserve(controls.size());
-    for (std::size_t s = 0; s < controls.size(); ++s) r.controls.push_back(model.allocate_control_adjoint(context));
-    StateAdjoint running = model.allocate_state_adjoint(context);
-    model.copy_state_adjoint(trajectory_adjoint.back(), running, context);
-    std::optional<State> recomputed_state;
-    std::optional<StepCache> recomputed_cache;
-    if (trajectory.tape_mode == TapeMode::recompute_step_cache && !controls.empty()) {
-        recomputed_state.emplace(model.allocate_state(context));
-        recomputed_cache.emplace(model.allocate_step_cache(context));
+    using ControlAdjoint = decltype(model.allocate_control_adjoint(context));
+    const std::size_t n = controls.size();
+    std::vector<ControlAdjoint> control_adjoints =
+        detail::make_n<ControlAdjoint>(n, [&] { return model.allocate_control_adjoint(context); });
+    auto parameter_adjoint = model.allocate_parameter_adjoint(context);
+    StateAdjoint later = model.allocate_state_adjoint(context), earlier = model.allocate_state_adjoint(context);
+    model.copy_state_adjoint(trajectory_adjoint[n], later, context);
+    detail::CacheReplay replay(model, context, trajectory, controls, parameters);
+    for (std::size_t remaining = n; remaining > 0; --remaining) {
+        const std::size_t step = remaining - 1;
+        const StepCache& cache = replay(step);
+        model.copy_state_adjoint(trajectory_adjoint[step], earlier, context);
+        model.vjp_step(trajectory.states[step], controls[step], parameters, trajectory.states[step + 1], cache, later, earlier,
+                       control_adjoints[step], parameter_adjoint, context);
+        std::swap(later, earlier);
     }
-    for (std::size_t step = controls.size(); step-- > 0;) {
-        const StepCache* cache;
-        if (trajectory.tape_mode == TapeMode::store_all) cache = &trajectory.step_caches[step];
-        else {
-            model.forward_step(trajectory.states[step], controls[step], parameters, *recomputed_state, *recomputed_cache, context);
-            cache = &*recomputed_cache;
-        }
-        StateAdjoint previous = model.allocate_state_adjoint(context);
-        model.vjp_step(trajectory.states[step], controls[step], parameters, trajectory.states[step + 1], *cache, running, previous,
-                       r.controls[step], r.parameters, context);
-        model.accumulate_state_adjoint(trajectory_adjoint[step], previous, context);
-        running = std::move(previous);
-    }
-    r.initial_state = std::move(running);
-    return r;
+    return {std::move(later), std::move(control_adjoints), std::move(parameter_adjoint)};
 }
 
-// solver::jvp, solver.ixx:83-113: the state tangent after every step of a stored trajectory
-// (element 0 is a copy of the initial tangent).
+// solver::jvp (solver.ixx:83-113): the state tangent after every step of a stored trajectory
+// (element 0 is a copy of the initial tangent); all n + 1 tangents are allocated up front and
+// jvp_step writes tangent step + 1 from tangent step.
 template <typename Model, typename Context, typename State, typename StepCache, typename Control, typename Parameters, typename StateTangent,
           typename ControlTangent, typename ParameterTangent>
     requires JvpModel<Model, Context, State, Control, Parameters>
 [[nodiscard]] auto jvp(const Model& model, Context& context, const Trajectory<State, StepCache>& trajectory, std::span<const Control> controls,
                        const Parameters& parameters, const StateTangent& initial_state_tangent,
                        std::span<const ControlTangent> control_tangents, const ParameterTangent& parameter_tangent) -> std::vector<StateTangent> {
-    std::vector<StateTangent> result;
-    result.reserve(trajectory.states.size());
-    StateTangent first = model.allocate_state_tangent(context);
-    model.copy_state_tangent(initial_state_tangent, first, context);
-    result.push_back(std::move(first));
-    std::optional<State> recomputed_state;
-    std::optional<StepCache> recomputed_cache;
-    if (trajectory.tape_mode == TapeMode::recompute_step_cache && !controls.empty()) {
-        recomputed_state.emplace(model.allocate_state(context));
-        recomputed_cache.emplace(model.allocate_step_cache(context));
-    }
-    for (std::size_t step = 0; step < controls.size(); ++step) {
-        const StepCache* cache;
-        if (trajectory.tape_mode == TapeMode::store_all) cache = &trajectory.step_caches[step];
-        else {
-            model.forward_step(trajectory.states[step], controls[step], parameters, *recomputed_state, *recomputed_cache, context);
-            cache = &*recomputed_cache;
-        }
-        StateTangent next = model.allocate_state_tangent(context);
-        model.jvp_step(trajectory.states[step], controls[step], parameters, trajectory.states[step + 1], *cache, result.back(),
-                       control_tangents[step], parameter_tangent, next, context);
-        result.push_back(std::move(next));
-    }
-    return result;
+    const std::size_t n = controls.size();
+    std::vector<StateTangent> tangents = detail::make_n<StateTangent>(n + 1, [&] { return model.allocate_state_tangent(context); });
+    model.copy_state_tangent(initial_state_tangent, tangents.front(), context);
+    detail::CacheReplay replay(model, context, trajectory, controls, parameters);
+    for (std::size_t step = 0; step < n; ++step)
+        model.jvp_step(trajectory.states[step], controls[step], parameters, trajectory.states[step + 1], replay(step), tangents[step],
+                       control_tangents[step], parameter_tangent, tangents[step + 1], context);
+    return tangents;
 }
 
 // Forward mode in one sweep (new): each step runs forward_step and then jvp_step on that step's
