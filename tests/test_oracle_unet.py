@@ -1,0 +1,213 @@
+"""Pins the warm-start network's FP64 oracle (oracle/unet_oracle.cpp).
+
+Each layer and the whole network are checked against PyTorch's CPU FP64 convolution, transposed
+convolution and max pooling (an independent implementation of the same definitions) and their
+autograd gradients; plus identities that hold whatever the implementation: a centred delta
+kernel is the identity, a shifted one a shift with zero fill; the loss of the exact error is 0
+at any scale; one Adam step moves each parameter by lr * sign(gradient).
+"""
+import numpy as np
+import pytest
+
+import unet as U
+
+torch = pytest.importorskip("torch")
+F = torch.nn.functional
+
+
+def t_ndhwc(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).permute(0, 4, 1, 2, 3).contiguous()
+
+
+def from_t(t):
+    return t.permute(0, 2, 3, 4, 1).contiguous().numpy()
+
+
+@pytest.fixture(scope="module")
+def rng():
+    return np.random.default_rng(5)
+
+
+def test_conv3_matches_torch(rng):
+    x = rng.standard_normal((2, 4, 6, 8, 5))
+    w = rng.standard_normal((7, 5, 3, 3, 3))
+    b = rng.standard_normal(7)
+    out = U.conv3_forward(x, w, b, relu=False)
+    ref = from_t(F.conv3d(t_ndhwc(x), torch.from_numpy(w), torch.from_numpy(b), padding=1))
+    np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(U.conv3_forward(x, w, b, relu=True), np.maximum(ref, 0.0), rtol=1e-12, atol=1e-12)
+
+
+def test_conv3_gradients_match_torch_autograd(rng):
+    x = rng.standard_normal((2, 4, 6, 8, 5))
+    w = rng.standard_normal((7, 5, 3, 3, 3))
+    g = rng.standard_normal((2, 4, 6, 8, 7))
+    xt = t_ndhwc(x).requires_grad_(True)
+    wt = torch.from_numpy(w).requires_grad_(True)
+    bt = torch.zeros(7, dtype=torch.float64, requires_grad=True)
+    (F.conv3d(xt, wt, bt, padding=1) * t_ndhwc(g)).sum().backward()
+    np.testing.assert_allclose(U.conv3_dgrad(g, w), from_t(xt.grad), rtol=1e-12, atol=1e-11)
+    dw, db = U.conv3_wgrad(x, g)
+    np.testing.assert_allclose(dw, wt.grad.numpy(), rtol=1e-12, atol=1e-11)
+    np.testing.assert_allclose(db, bt.grad.numpy(), rtol=1e-12, atol=1e-11)
+
+
+def test_conv3_delta_kernels():
+    """Centre tap = identity; the tap (dx, dy, dz) = (1, 0, -1) reads in[x + 1, y, z - 1] with 0
+    beyond the edges (no library involved)."""
+    x = np.arange(2 * 3 * 4 * 5 * 2, dtype=np.float64).reshape(2, 3, 4, 5, 2)
+    w = np.zeros((2, 2, 3, 3, 3))
+    w[0, 0, 1, 1, 1] = w[1, 1, 1, 1, 1] = 1.0
+    np.testing.assert_array_equal(U.conv3_forward(x, w), x)
+    w = np.zeros((2, 2, 3, 3, 3))
+    w[0, 0, 0, 1, 2] = w[1, 1, 0, 1, 2] = 1.0     # kz = 0 (dz = -1), ky = 1, kx = 2 (dx = +1)
+    out = U.conv3_forward(x, w)
+    ref = np.zeros_like(x)
+    ref[:, 1:, :, :-1, :] = x[:, :-1, :, 1:, :]
+    np.testing.assert_array_equal(out, ref)
+
+
+def test_tconv2_matches_torch(rng):
+    x = rng.standard_normal((2, 2, 3, 4, 6))
+    w = rng.standard_normal((6, 5, 2, 2, 2))
+    b = rng.standard_normal(5)
+    out = U.tconv2_forward(x, w, b)
+    ref = from_t(F.conv_transpose3d(t_ndhwc(x), torch.from_numpy(w), torch.from_numpy(b), stride=2))
+    np.testing.assert_allclose(out, ref, rtol=1e-12, atol=1e-12)
+    g = rng.standard_normal(out.shape)
+    xt = t_ndhwc(x).requires_grad_(True)
+    wt = torch.from_numpy(w).requires_grad_(True)
+    bt = torch.zeros(5, dtype=torch.float64, requires_grad=True)
+    (F.conv_transpose3d(xt, wt, bt, stride=2) * t_ndhwc(g)).sum().backward()
+    np.testing.assert_allclose(U.tconv2_dgrad(g, w), from_t(xt.grad), rtol=1e-12, atol=1e-11)
+    dw, db = U.tconv2_wgrad(x, g)
+    np.testing.assert_allclose(dw, wt.grad.numpy(), rtol=1e-12, atol=1e-11)
+    np.testing.assert_allclose(db, bt.grad.numpy(), rtol=1e-12, atol=1e-11)
+
+
+def test_maxpool_matches_torch_and_routes_ties_to_the_first(rng):
+    x = rng.standard_normal((2, 4, 6, 8, 3))
+    out = U.maxpool_forward(x)
+    xt = t_ndhwc(x).requires_grad_(True)
+    ref = F.max_pool3d(xt, 2)
+    np.testing.assert_array_equal(out, from_t(ref.detach()))
+    g = rng.standard_normal(out.shape)
+    (ref * t_ndhwc(g)).sum().backward()
+    np.testing.assert_allclose(U.maxpool_backward(x, g), from_t(xt.grad), rtol=0, atol=0)
+    # a tie: the whole group equal -> the gradient goes to t = 0, the voxel (2x, 2y, 2z)
+    x = np.ones((1, 2, 2, 2, 1))
+    din = U.maxpool_backward(x, np.full((1, 1, 1, 1, 1), 3.0))
+    assert din[0, 0, 0, 0, 0] == 3.0 and din.sum() == 3.0
+    x[0, 1, 1, 1, 0] = 2.0   # the last voxel (t = 7) is the unique maximum
+    din = U.maxpool_backward(x, np.full((1, 1, 1, 1, 1), 3.0))
+    assert din[0, 1, 1, 1, 0] == 3.0 and din.sum() == 3.0
+
+
+def torch_unet():
+    import sys
+    import os
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "diff-solver_amd"))
+    from ds_smoke.warm_start import build_unet
+
+    return build_unet().double()
+
+
+def test_network_forward_and_gradient_match_torch(rng):
+    model = torch_unet()
+    assert U.parameter_count() == sum(p.numel() for p in model.parameters()) == 339681
+    flat = np.concatenate([p.detach().numpy().ravel() for p in model.state_dict().values()])
+    x2 = rng.standard_normal((2, 4, 8, 8, 2))
+    x2[..., 1] = (x2[..., 1] > 0).astype(np.float64)
+    dout = rng.standard_normal((2, 4, 8, 8))
+    out, grad = U.network(flat, x2, dout=dout)
+    xt = t_ndhwc(x2)
+    ref = model(xt)[:, 0]
+    np.testing.assert_allclose(out, ref.detach().numpy(), rtol=1e-10, atol=1e-12)
+    (ref * torch.from_numpy(dout)).sum().backward()
+    gref = np.concatenate([p.grad.numpy().ravel() for p in model.parameters()])
+    np.testing.assert_allclose(grad, gref, rtol=1e-9, atol=1e-11)
+
+
+def test_network_bf16_mode_stays_close(rng):
+    """The BF16-rounded forward differs from FP64 by the rounding of ~14 layers only."""
+    model = torch_unet()
+    flat = np.concatenate([p.detach().numpy().ravel() for p in model.state_dict().values()])
+    x2 = rng.standard_normal((1, 4, 8, 8, 2))
+    out = U.network(flat, x2)
+    outb = U.network(flat, x2, bf16=True)
+    e = np.linalg.norm(outb - out) / np.linalg.norm(out)
+    assert 1e-5 < e < 5e-2, e
+
+
+def closed_code(nz, ny, nx):
+    """Diagonal counts of a closed box's pressure unknowns (neighbours inside the box) with the
+    anchor cell 0 removed: its neighbours keep their count, their coupling to it is dropped."""
+    code = np.zeros((nz, ny, nx), dtype=np.uint8)
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                code[z, y, x] = (x > 0) + (x < nx - 1) + (y > 0) + (y < ny - 1) + (z > 0) + (z < nz - 1)
+    code[0, 0, 0] = 0
+    return code
+
+
+def dense_matrix(code):
+    nz, ny, nx = code.shape
+    n = code.size
+    A = np.zeros((n, n))
+    idx = lambda x, y, z: x + nx * (y + ny * z)
+    for z in range(nz):
+        for y in range(ny):
+            for x in range(nx):
+                i = idx(x, y, z)
+                if not code[z, y, x]:
+                    continue
+                A[i, i] = code[z, y, x]
+                for dx, dy, dz in ((1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1)):
+                    xx, yy, zz = x + dx, y + dy, z + dz
+                    if 0 <= xx < nx and 0 <= yy < ny and 0 <= zz < nz and code[zz, yy, xx]:
+                        A[i, idx(xx, yy, zz)] = -1.0
+    return A
+
+
+def test_energy_loss_zero_for_exact_error_and_gradient_by_differences(rng):
+    code = closed_code(3, 4, 5)
+    A = dense_matrix(code)
+    unk = code.reshape(-1) > 0
+    Au = A[np.ix_(unk, unk)]
+    e = np.zeros((2, code.size))
+    e[:, unk] = rng.standard_normal((2, unk.sum()))
+    r = np.zeros_like(e)
+    r[:, unk] = e[:, unk] @ Au.T
+    shp = (2,) + code.shape
+    loss, _ = U.energy_loss(code, 3.0 * e.reshape(shp), r.reshape(shp), e.reshape(shp))
+    np.testing.assert_allclose(loss, 0.0, atol=1e-12)
+    g = np.zeros_like(e)
+    g[:, unk] = rng.standard_normal((2, unk.sum()))
+    loss, dg = U.energy_loss(code, g.reshape(shp), r.reshape(shp), e.reshape(shp))
+    # closed form with the dense matrix
+    for b in range(2):
+        q, D, E = g[b] @ r[b], g[b] @ A @ g[b], e[b] @ r[b]
+        np.testing.assert_allclose(loss[b], 1 - q * q / (D * E), rtol=1e-12)
+    assert np.all((0.0 <= loss) & (loss <= 1.0))
+    eps = 1e-6
+    for i in np.flatnonzero(unk)[:6]:
+        gp, gm = g.copy(), g.copy()
+        gp[:, i] += eps
+        gm[:, i] -= eps
+        fd = (U.energy_loss(code, gp.reshape(shp), r.reshape(shp), e.reshape(shp))[0] -
+              U.energy_loss(code, gm.reshape(shp), r.reshape(shp), e.reshape(shp))[0]) / (2 * eps)
+        np.testing.assert_allclose(dg.reshape(2, -1)[:, i], fd, rtol=1e-6, atol=1e-10)
+
+
+def test_adam_first_step_is_lr_times_sign():
+    p = np.array([1.0, -2.0, 3.0, 0.5])
+    g = np.array([0.3, -4.0, 1e-3, -2e-2])
+    m, v = np.zeros(4), np.zeros(4)
+    p0 = p.copy()
+    U.adam(p, g, m, v, 1, lr=0.01)
+    np.testing.assert_allclose(p, p0 - 0.01 * np.sign(g), rtol=0, atol=1e-7)
+    # second step with the same gradient: still lr * sign (m_hat = g, v_hat = g^2)
+    U.adam(p, g, m, v, 2, lr=0.01)
+    np.testing.assert_allclose(p, p0 - 0.02 * np.sign(g), rtol=0, atol=2e-7)
