@@ -3,13 +3,15 @@
 
 Workload (one bench step): the 128^3 smoke plume, 64 forward steps, then the VJP of
 0.5 ||rho_64 - target||^2 with respect to the initial state by a reverse sweep with a checkpoint
-every 8 steps (recompute in between). Each warm-start mode of the pressure solves (x0 = 0, the
-previous solve's solution, the linear extrapolation of the previous two solutions, the warm-start
-network) runs in turn; value = simulated fwd+VJP steps
-per second of the fastest, over the timed bench steps, inputs resident in HBM. Also reported: PCG
-iterations per solve (forward, recompute, adjoint) for every mode, the HBM roofline of the
-dominant kernel (HIP-event timing of sampled launches inside the timed region), and the FP64 CPU
-oracle timed on the host cores on a bounded sample.
+every 8 steps (recompute in between). Each warm-start mode of the pressure solves runs in turn in
+its own context: x0 = 0 ("plain"), the sequence guess (quadratic extrapolation of the forward
+solves, previous adjoint solution), the warm-start network alone, and the network correcting the
+sequence guess ("cnn", the headline when its weights exist). value = simulated fwd+VJP steps per
+second of the headline mode over the timed bench steps, inputs resident in HBM. Also reported: PCG
+iterations per solve (forward, recompute, adjoint) for every mode, the network-vs-plain
+comparison, the HBM roofline of the dominant kernel (HIP-event timing of sampled launches inside
+the timed region), the MFMA roofline of the network's inference, and the FP64 CPU oracle timed on
+the host cores on a bounded sample.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -92,6 +94,43 @@ def calibrate_copy(mib):
                       "ms_per_copy": dt * 1e3, "gbs": 2 * n * 4 / dt / 1e9}))
 
 
+# Multiply-adds per fine voxel of one inference of the warm-start network (3-level U-Net,
+# 16/32/64 channels): level 1 (every voxel): 3x3x3 convolutions 2->16, 16->16, 32->16, 16->16, the
+# 32->16 transposed convolution (one 2x2x2 tap per fine voxel) and the 16->1 head; level 2 (one
+# voxel per 8): 16->32, 32->32, 64->32, 32->32 and the 64->32 transposed convolution; level 3
+# (one per 64): 32->64, 64->64.
+UNET_MACS_PER_VOXEL = (27 * (2 * 16 + 16 * 16 + 32 * 16 + 16 * 16) + 32 * 16 + 16
+                       + (27 * (16 * 32 + 32 * 32 + 64 * 32 + 32 * 32) + 64 * 32) / 8
+                       + 27 * (32 * 64 + 64 * 64) / 64)
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+
+
+def unet_timing(ctx, sc, repeats=20):
+    """MFMA roofline of the network's inference at the workload's grid: algorithmic FLOPs
+    (2 x UNET_MACS_PER_VOXEL per voxel) / the mean time of `repeats` inferences (HIP events on the
+    context's stream, after one warm-up)."""
+    import numpy as np
+    import torch
+
+    _, code = ctx.masks()
+    rng = np.random.default_rng(0)
+    b = ctx.to_device(np.where(code > 0, rng.standard_normal(code.shape), 0.0).astype(np.float32))
+    ctx.warm_start_infer(b)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ctx.synchronize()
+    start.record(ctx.stream)
+    for _ in range(repeats):
+        ctx.warm_start_guess(b)
+    stop.record(ctx.stream)
+    stop.synchronize()
+    ms = start.elapsed_time(stop) / repeats
+    flops = 2.0 * UNET_MACS_PER_VOXEL * int(np.prod(sc.resolution))
+    tf = flops / (ms * 1e-3) / 1e12
+    return {"bound": "mfma", "kernel": "warm-start network inference (13 layers)", "achieved": round(tf, 2),
+            "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4),
+            "flops_per_inference": flops, "ms_per_inference": round(ms, 4), "dtype": "bf16 (fp32 accumulation)"}
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch from the committed rocprofv3 counter summary, when present."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_plume128_traffic.json")
@@ -112,10 +151,11 @@ def main():
     ap.add_argument("--profile-every", type=int, default=8)
     ap.add_argument("--cpu-sample-steps", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--warm-start", choices=["all", "none", "previous", "extrapolate", "projection", "cnn"], default="all",
-                    help="initial guess of each pressure solve: zero, the previous solve's solution, the linear "
-                         "extrapolation of the previous two, the projection onto the previous eight, or the warm-start "
-                         "network; all = each in turn (the network when its weights exist), value = the fastest")
+    ap.add_argument("--warm-start", choices=["all", "none", "previous", "extrapolate", "projection", "cnn_alone", "cnn"], default="all",
+                    help="initial guess of each pressure solve: zero, the previous solve's solution, the extrapolated "
+                         "sequence, the projection onto the previous eight, the warm-start network alone, or the network "
+                         "correcting the extrapolated sequence; all = none, extrapolate, cnn_alone and cnn in turn (the "
+                         "network modes when its weights exist)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "diff-solver_amd", "data", "warm_start_plume.npz"),
                     help="warm-start network parameters (for --warm-start cnn)")
     ap.add_argument("--cg-variant", choices=["single", "two", "resident"], default="single",
@@ -140,20 +180,22 @@ def main():
     variant = {"single": abi.CG_SINGLE_REDUCTION, "two": abi.CG_TWO_REDUCTION,
                "resident": abi.CG_SINGLE_REDUCTION_RESIDENT}[args.cg_variant]
     if args.warm_start == "all":
-        modes = ["none", "previous", "extrapolate", "projection"] + (["cnn"] if os.path.exists(args.weights) else [])
+        modes = ["none", "extrapolate"] + (["cnn_alone", "cnn"] if os.path.exists(args.weights) else [])
     else:
         modes = [args.warm_start]
 
     def run_mode(mode):
         """Warm-up, then the timed bench steps of one warm-start mode in a fresh context."""
         warm = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS, "extrapolate": abi.WARM_START_EXTRAPOLATE,
-                "projection": abi.WARM_START_PROJECTION, "cnn": abi.WARM_START_CNN}[mode]
+                "projection": abi.WARM_START_PROJECTION, "cnn_alone": abi.WARM_START_CNN_ALONE, "cnn": abi.WARM_START_CNN}[mode]
         ctx = Context(sc.config(tolerance=1e-6, warm_start=warm, cg_variant=variant), abi.MODE_DIFFERENTIABLE)
-        if mode == "cnn":
+        unet = None
+        if mode in ("cnn", "cnn_alone"):
             from ds_smoke import warm_start
 
             flat, output_scale, _ = warm_start.load(args.weights)
             ctx.warm_start_load(flat, output_scale)
+            unet = unet_timing(ctx, sc)
         s0 = ctx.state_from_host(scenes.initial_state(sc))
         ctl = ctx.control_from_host(scenes.control(sc))
         params = ctx.allocate_parameters(sc.params)
@@ -196,7 +238,7 @@ def main():
         iters = (info.forward_iterations + info.recompute_iterations + info.adjoint_iterations) / args.sim_steps
         return {"value": args.steps * args.sim_steps / elapsed, "ms_per_step": elapsed / args.steps * 1e3, "loss": loss,
                 "per_solve": per_solve, "kernels": kern, "not_converged": info.not_converged,
-                "cells": n, "unknowns": nf, "cg_iterations_per_sim_step": iters}
+                "cells": n, "unknowns": nf, "cg_iterations_per_sim_step": iters, "unet": unet}
 
     results = {m: run_mode(m) for m in modes}
     # A mode whose solves did not all converge, or whose loss is not finite, is not a valid
@@ -226,7 +268,22 @@ def main():
     step_gbs = step_bytes / step_s / 1e9
     step_roofline = {"bound": "hbm", "bytes_per_step": round(step_bytes), "cg_iterations_per_step": round(r["cg_iterations_per_sim_step"], 1),
                      "achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(step_gbs / HBM_PEAK_GBS, 4)}
-    label = {"none": "plain", "previous": "previous", "extrapolate": "extrapolate", "projection": "projection", "cnn": "cnn"}
+    label = {"none": "plain", "previous": "previous", "extrapolate": "extrapolate", "projection": "projection", "cnn_alone": "cnn_alone",
+             "cnn": "cnn"}
+    # the metric's own comparison: the network's modes against x0 = 0 at the same 1e-6 residual
+    cnn_vs_plain = None
+    if "none" in results:
+        base = results["none"]
+        cnn_vs_plain = {}
+        for m in ("none", "cnn_alone", "extrapolate", "cnn"):
+            if m not in results:
+                continue
+            rm = results[m]
+            cnn_vs_plain[label[m]] = {
+                "steps_per_s": round(rm["value"], 4), "speedup_vs_plain": round(rm["value"] / base["value"], 4),
+                "forward_cg_per_solve": round(rm["per_solve"]["forward"], 1), "adjoint_cg_per_solve": round(rm["per_solve"]["adjoint"], 1),
+                "cg_iterations_vs_plain": round(rm["cg_iterations_per_sim_step"] / base["cg_iterations_per_sim_step"], 4),
+                "excluded": excluded.get(m)}
     result = {
         "metric": METRIC,
         "value": round(r["value"], 4),
@@ -247,12 +304,19 @@ def main():
         "cg_iterations_per_solve": {label[m]: {k: round(v, 1) for k, v in results[m]["per_solve"].items()} for m in modes},
         "steps_per_s_by_warm_start": {label[m]: round(results[m]["value"], 4) for m in modes},
         "loss": r["loss"],
+        "cnn_vs_plain": cnn_vs_plain,
         "roofline": roofline,
+        "unet_roofline": next((results[m]["unet"] for m in ("cnn", "cnn_alone") if m in results and results[m]["unet"]), None),
         "step_roofline": step_roofline,
         "kernels": {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in kern.items()},
     }
     if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline(args.grid, args.sim_steps, args.checkpoint, args.cpu_sample_steps)
+        cb = cpu_baseline(args.grid, args.sim_steps, args.checkpoint, args.cpu_sample_steps)
+        if "none" in results:
+            # the like-for-like ratio: the GPU from x0 = 0 (the oracle starts every solve from 0)
+            cb["sample"] += (f"; against the GPU from x0 = 0 ({results['none']['value']:.2f} steps/s): "
+                             f"{results['none']['value'] / cb['value']:.0f}x")
+        result["cpu_baseline"] = cb
     print(json.dumps(result))
 
 

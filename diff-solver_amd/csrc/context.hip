@@ -90,12 +90,17 @@ void validate(const ds_config& cfg, uint32_t mode) {
     for (uint32_t k = 0; k < cfg.collider_count; ++k) {
         const ds_collider& col = cfg.colliders[k];
         if (col.shape > 1u) bad("collider shape out of range");
-        for (int a = 0; a < 3; ++a)
-            if (!(col.size[a] > 0.0F)) bad("collider size must be positive");
+        // an ellipsoid divides by its radii (model.cpp:17-22); a box's half extent may be 0 (it
+        // then holds the cells whose centre lies on its mid-plane, model.cpp:24-26)
+        for (int a = 0; a < 3; ++a) {
+            if (!std::isfinite(col.size[a])) bad("collider size must be finite");
+            if (col.shape == DS_SHAPE_ELLIPSOID && !(col.size[a] > 0.0F)) bad("ellipsoid radii must be positive");
+            if (col.shape == DS_SHAPE_BOX && !(col.size[a] >= 0.0F)) bad("box half extents must be non-negative");
+        }
     }
     if (cfg.projection_solver > DS_PROJECTION_RBGS) bad("projection_solver out of range");
-    if (cfg.warm_start > DS_WARM_START_PROJECTION) bad("warm_start out of range");
-    if (cfg.warm_start == DS_WARM_START_CNN && (cfg.resolution[0] % 64 || cfg.resolution[1] % 16 || cfg.resolution[2] % 4))
+    if (cfg.warm_start > DS_WARM_START_CNN_ALONE) bad("warm_start out of range");
+    if ((cfg.warm_start == DS_WARM_START_CNN || cfg.warm_start == DS_WARM_START_CNN_ALONE) && (cfg.resolution[0] % 64 || cfg.resolution[1] % 16 || cfg.resolution[2] % 4))
         throw Failure{DS_ERROR_UNSUPPORTED, "the warm-start network needs nx % 64 == 0, ny % 16 == 0 and nz % 4 == 0"};
     if (!(cfg.cg_tolerance >= 0.0) || !std::isfinite(cfg.cg_tolerance)) bad("cg_tolerance must be >= 0 and finite");
     if (cfg.cg_variant > DS_CG_SINGLE_REDUCTION_RESIDENT) bad("cg_variant out of range");
@@ -392,7 +397,7 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
             c->advected[a] = device_alloc<float>(c, g.nf[a]);
         }
         c->pressure_hi = device_alloc<float>(c, n);
-        if (cfg.warm_start == DS_WARM_START_EXTRAPOLATE) {
+        if (cfg.warm_start == DS_WARM_START_EXTRAPOLATE || cfg.warm_start == DS_WARM_START_CNN) {
             c->pressure_prev = device_alloc<float>(c, 2 * (size_t)n);
             c->guess = device_alloc<float>(c, n);
         }
@@ -541,6 +546,19 @@ ds_status ds_context_domain_masks(ds_context* c, uint8_t* mask, uint8_t* code) {
         if (mask) DS_HIP(hipMemcpyAsync(mask, c->mask, c->geo.n, hipMemcpyDeviceToHost, c->stream));
         if (code) DS_HIP(hipMemcpyAsync(code, c->code, c->geo.n, hipMemcpyDeviceToHost, c->stream));
         DS_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+ds_status ds_context_last_solution(ds_context* c, uint32_t which, float* out) {
+    if (!c || !out || which > 1) {
+        set_error("ds_context_last_solution: null argument or unknown sequence");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] {
+        const bool valid = which == 0 ? c->pressure_valid : c->lambda_valid;
+        require(valid && (which == 0 || c->lambda_hi), "ds_context_last_solution: no solve of that sequence yet");
+        combine_pair(c, which == 0 ? c->pressure_hi : c->lambda_hi, which == 0 ? c->pressure_lo : c->lambda_lo, out);
+        check_launch("combine_pair_kernel");
     });
 }
 

@@ -280,8 +280,9 @@ ds_solve_info pcg_solve(ds_context* c, float* x_hi, float* x_lo, const float* x0
 // Solves A x = b with b built from `u` (kind 1: forward divergence of the advected velocity;
 // kind 2: adjoint, u = projected-velocity adjoint plus `extra`; kind 3: tangent, the forward
 // divergence without the fixed-face term) or given in c->pcg.b (kind 0).
+// correct: under DS_WARM_START_CNN the network corrects the given guess (x0h, x0l).
 ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], const float* const extra[3], float* xh,
-                             float* xl, const float* x0h, const float* x0l, double tol, uint32_t max_it);
+                             float* xl, const float* x0h, const float* x0l, double tol, uint32_t max_it, bool correct = false);
 uint32_t pcg_periodic_bits(const ds_context* c);
 // DS_WARM_START_PROJECTION (window.hip): the window of a right-hand-side kind (null when the mode
 // is off or the kind has none), the guess for right-hand side b (null while the window is empty),
@@ -307,8 +308,9 @@ void launch_apply_matrix(ds_context* c, const float* x, float* y);
 // Warm-start network (unet.hip).
 uint64_t unet_parameter_count();
 void unet_load(ds_context* c, const float* params, uint64_t count, double output_scale);
-// Guess for A x = b from b (device); needs the ||b||^2 partials of b in partial set 0.
-const float* unet_infer(ds_context* c, const float* b);
+// The network's guess for A x = b (device b), corrected from the base guess xb when not null.
+const float* unet_infer(ds_context* c, const float* b, const float* xb);
+bool unet_loaded(const ds_context* c);
 void unet_release(UNet* u);
 
 }  // namespace ds

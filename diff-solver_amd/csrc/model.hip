@@ -104,7 +104,8 @@ __global__ __launch_bounds__(256) void extrapolate_kernel(uint32_t n, int order,
 std::pair<const float*, const float*> sequence_guess(ds_context* c, float* hi, float* lo, bool valid, float* prev, int& prev_count,
                                                      bool previous_under_network = false) {
     uint32_t mode = c->config.warm_start;
-    if (previous_under_network && mode == DS_WARM_START_CNN) mode = DS_WARM_START_PREVIOUS;
+    // the network corrects the extrapolated forward guess and the previous adjoint solution
+    if (mode == DS_WARM_START_CNN) mode = previous_under_network ? DS_WARM_START_PREVIOUS : DS_WARM_START_EXTRAPOLATE;
     if (!valid || (mode != DS_WARM_START_PREVIOUS && mode != DS_WARM_START_EXTRAPOLATE)) return {nullptr, nullptr};
     if (mode == DS_WARM_START_EXTRAPOLATE && prev) {
         const int order = prev_count;
@@ -164,7 +165,8 @@ ds_solve_info forward_step(ds_context* c, const ds_state* s, const ds_control* k
         x0l = g.second;
     }
     const float* uac[3]{ua.p[0], ua.p[1], ua.p[2]};
-    const ds_solve_info si = pcg_solve_kind(c, 1, uac, nullptr, c->pressure_hi, c->pressure_lo, x0h, x0l, c->tolerance, c->max_iterations);
+    const ds_solve_info si = pcg_solve_kind(c, 1, uac, nullptr, c->pressure_hi, c->pressure_lo, x0h, x0l, c->tolerance, c->max_iterations,
+                                            c->config.warm_start == DS_WARM_START_CNN && !cache->pressure_guess);
     c->pressure_valid = true;
     launch_project(c, CF3{{ua.p[0], ua.p[1], ua.p[2]}}, c->pressure_hi, c->pressure_lo, F3{{next->velocity[0], next->velocity[1], next->velocity[2]}});
     launch_advect_scalars(c, 2, 0, cache->sourced_density, cache->sourced_temperature,
@@ -211,11 +213,11 @@ ds_solve_info vjp_step(ds_context* c, const ds_parameters* p, const ds_state* st
     const float* ex[3]{c->projected_extra[0], c->projected_extra[1], c->projected_extra[2]};
     // adjoint solves start from the previous adjoint solution under both sequence modes (on the
     // 128^3 plume the extrapolated adjoint guess took more iterations, 606 per solve, than the
-    // previous solution, 566) and under the network, which is trained on forward pairs only (its
-    // guess cost 923 iterations per adjoint solve against 804 from x0 = 0)
+    // previous solution, 566); under the network, the network corrects that solution
     int no_prev = 0;
     const auto lg = sequence_guess(c, c->lambda_hi, c->lambda_lo, c->lambda_valid, nullptr, no_prev, true);
-    const ds_solve_info si = pcg_solve_kind(c, 2, na, ex, c->lambda_hi, c->lambda_lo, lg.first, lg.second, c->tolerance, c->max_iterations);
+    const ds_solve_info si = pcg_solve_kind(c, 2, na, ex, c->lambda_hi, c->lambda_lo, lg.first, lg.second, c->tolerance, c->max_iterations,
+                                            c->config.warm_start == DS_WARM_START_CNN);
     c->lambda_valid = true;
     launch_projection_vjp_faces(c, na, ex, c->lambda_hi, c->lambda_lo, F3{{c->raw_adj[0], c->raw_adj[1], c->raw_adj[2]}}, true);
     launch_advect_velocity_vjp(c, CF3{{cache->forced_velocity[0], cache->forced_velocity[1], cache->forced_velocity[2]}},

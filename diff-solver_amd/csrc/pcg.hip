@@ -608,7 +608,7 @@ void pcg_resident_prepare(ds_context* c) {
 
 // Solves A x = b for the given right-hand-side kind. Returns the solve info; x = (xh, xl).
 ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], const float* const extra[3], float* xh,
-                             float* xl, const float* x0h, const float* x0l, double tol, uint32_t max_it) {
+                             float* xl, const float* x0h, const float* x0l, double tol, uint32_t max_it, bool correct) {
     if (c->config.projection_solver == DS_PROJECTION_RBGS && kind != kRhsGiven) return rbgs_solve_kind(c, kind, u, extra, xh, xl);
     RhsIn in{};
     for (int a = 0; a < 3; ++a) {
@@ -626,10 +626,21 @@ ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], c
         x0h = window_guess(c, *win, c->pcg.b);
         x0l = nullptr;
         kind = kRhsGiven;
-    } else if (x0h == nullptr && c->config.warm_start == DS_WARM_START_CNN) {
-        // the network's guess: b first (set-up without a guess), then the set-up again from the guess
+    } else if ((c->config.warm_start == DS_WARM_START_CNN || c->config.warm_start == DS_WARM_START_CNN_ALONE) &&
+               (x0h == nullptr || (correct && c->config.warm_start == DS_WARM_START_CNN))) {
+        // the network's guess: b first (set-up without a guess), then the set-up again from the guess;
+        // a given guess (the sequence's) is the base the network corrects, combined to FP32 first
+        const float* base = nullptr;
+        if (x0h) {
+            if (x0l) {
+                combine_pair(c, x0h, x0l, c->guess);
+                base = c->guess;
+            } else {
+                base = x0h;
+            }
+        }
         launch_setup(c, kind, in, nullptr, nullptr, false, xh, xl);
-        x0h = unet_infer(c, c->pcg.b);
+        x0h = unet_infer(c, c->pcg.b, base);
         x0l = nullptr;
         kind = kRhsGiven;
         network_guess = true;
@@ -775,7 +786,19 @@ ds_status ds_warm_start_infer(ds_context* c, const float* b, float* x) {
         launch_setup(c, kRhsGiven, in, nullptr, nullptr, false, c->pressure_hi, c->pressure_lo);   // ||b||^2 partials
         c->pressure_valid = false;
         c->pressure_prev_count = 0;
-        const float* g = unet_infer(c, c->pcg.b);
+        const float* g = unet_infer(c, c->pcg.b, nullptr);
+        DS_HIP(hipMemcpyAsync(x, g, (size_t)c->geo.n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
+        DS_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+ds_status ds_warm_start_guess(ds_context* c, const float* b, const float* base, float* x) {
+    if (!c || !b || !x) {
+        set_error("ds_warm_start_guess: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] {
+        const float* g = unet_infer(c, b, base);
         DS_HIP(hipMemcpyAsync(x, g, (size_t)c->geo.n * sizeof(float), hipMemcpyDeviceToDevice, c->stream));
         DS_HIP(hipStreamSynchronize(c->stream));
     });

@@ -28,10 +28,10 @@ def run(grid: int = 128, iterations: int = 200, steps: int = 64, k: int = 8, lr:
     from .solver import Context
 
     modes = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS, "extrapolate": abi.WARM_START_EXTRAPOLATE, "projection": abi.WARM_START_PROJECTION,
-             "cnn": abi.WARM_START_CNN}
+             "cnn_alone": abi.WARM_START_CNN_ALONE, "cnn": abi.WARM_START_CNN}
     sc = scenes.plume(grid, target_seed=target_seed)
     ctx = Context(sc.config(tolerance=tolerance, warm_start=modes[warm_start]), abi.MODE_DIFFERENTIABLE, device)
-    if warm_start == "cnn":
+    if warm_start in ("cnn", "cnn_alone"):
         from . import warm_start as ws
 
         params_flat, output_scale, _ = ws.load(weights)
@@ -73,7 +73,7 @@ def run(grid: int = 128, iterations: int = 200, steps: int = 64, k: int = 8, lr:
     per_solve = {key: (n_it / n_s if n_s else 0.0) for key, (n_s, n_it) in solves.items()}
     ctx.close()
     return {"grid": grid, "iterations": iterations, "steps": steps, "checkpoint_interval": k, "lr": lr, "warm_start": warm_start,
-            "initial_loss": history[0], "final_loss": history[-1], "loss_every_10": history[::10] + [history[-1]],
+            "initial_loss": history[0], "final_loss": history[-1], "loss_every_10": history[::10] + [history[-1]], "losses": history,
             "wall_seconds": wall, "seconds_per_iteration": wall / iterations, "cg_iterations_per_solve": per_solve}
 
 
@@ -84,12 +84,12 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=64)
     ap.add_argument("--k", type=int, default=8)
     ap.add_argument("--lr", type=float, default=1e-2)
-    ap.add_argument("--warm-start", choices=["none", "previous", "extrapolate", "projection", "cnn"], default="none")
+    ap.add_argument("--warm-start", choices=["none", "previous", "extrapolate", "projection", "cnn_alone", "cnn"], default="none")
     ap.add_argument("--weights", default=None, help="warm-start network (a file written by ds_smoke.train_warm_start)")
     ap.add_argument("--report", type=int, default=10)
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
-    if args.warm_start == "cnn" and not args.weights:
+    if args.warm_start in ("cnn", "cnn_alone") and not args.weights:
         import os
 
         args.weights = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "data", "warm_start_plume.npz")

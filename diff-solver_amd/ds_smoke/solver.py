@@ -38,6 +38,7 @@ def load_library(path: str | None = None):
         raise RuntimeError(f"HIP smoke library not found at {path}; run __graft_entry__.build() first")
     L = C.CDLL(path)
     P = C.c_void_p
+    U32 = C.c_uint32
     sigs = {
         "ds_last_error": (C.c_char_p, []),
         "ds_config_cell_count": (C.c_uint64, [C.POINTER(abi.Config)]),
@@ -92,6 +93,7 @@ def load_library(path: str | None = None):
                                          C.POINTER(abi.Parameters), C.c_uint32, C.c_uint32, P, C.POINTER(abi.State),
                                          C.POINTER(abi.StateAdjoint), C.POINTER(abi.ControlAdjoint),
                                          C.POINTER(abi.ParameterAdjoint), C.POINTER(abi.TrajectoryInfo)]),
+        "ds_context_last_solution": (C.c_int32, [P, C.c_uint32, P]),
         "ds_warm_start_parameter_count": (C.c_uint64, []),
         "ds_warm_start_load": (C.c_int32, [P, P, C.c_uint64, C.c_double]),
         "ds_warm_start_infer": (C.c_int32, [P, P, P]),
@@ -117,6 +119,23 @@ def load_library(path: str | None = None):
                                           C.POINTER(abi.ParameterAdjoint), C.POINTER(abi.TrajectoryInfo)]),
         "ds_example_write_control": (C.c_int32, [P, C.c_uint64, C.POINTER(abi.JetControl), P, P, C.POINTER(P)]),
         "ds_example_reduce_metrics": (C.c_int32, [P, P, P, C.POINTER(P), C.POINTER(P), P]),
+        "ds_warm_start_parameter_count": (C.c_uint64, []),
+        "ds_warm_start_load": (C.c_int32, [P, P, C.c_uint64, C.c_double]),
+        "ds_warm_start_infer": (C.c_int32, [P, P, P]),
+        "ds_warm_start_guess": (C.c_int32, [P, P, P, P]),
+        "ds_warm_start_pair": (C.c_int32, [P, P, P, P]),
+        "ds_unet_trainer_create": (C.c_int32, [P, C.c_uint32, P, C.c_uint64, C.POINTER(P)]),
+        "ds_unet_trainer_destroy": (C.c_int32, [P]),
+        "ds_unet_trainer_step": (C.c_int32, [P, P, C.c_uint32, C.c_float, P]),
+        "ds_unet_trainer_read": (C.c_int32, [P, C.c_uint32, P]),
+        "ds_unet_conv3_forward": (C.c_int32, [U32] * 8 + [P] * 5 + [P]),
+        "ds_unet_conv3_dgrad": (C.c_int32, [U32] * 7 + [P] * 6),
+        "ds_unet_conv3_wgrad": (C.c_int32, [U32] * 7 + [P] * 6),
+        "ds_unet_tconv2_forward": (C.c_int32, [U32] * 6 + [P] * 5),
+        "ds_unet_tconv2_dgrad": (C.c_int32, [U32] * 6 + [P] * 5),
+        "ds_unet_tconv2_wgrad": (C.c_int32, [U32] * 6 + [P] * 5),
+        "ds_unet_maxpool_forward": (C.c_int32, [U32] * 5 + [P] * 3),
+        "ds_unet_maxpool_backward": (C.c_int32, [U32] * 5 + [P] * 5),
     }
     for name, (res, args) in sigs.items():
         fn = getattr(L, name)
@@ -536,6 +555,25 @@ class Context:
         x = self.zeros_cell()
         self._check(self.lib.ds_warm_start_infer(self.h, _ptr(b), _ptr(x)), "ds_warm_start_infer")
         return x
+
+    def last_solution(self, which: int, out=None):
+        """The last forward (0) or adjoint (1) solution as an FP32 cell field."""
+        x = self.zeros_cell() if out is None else out
+        self._check(self.lib.ds_context_last_solution(self.h, int(which), _ptr(x)), "ds_context_last_solution")
+        return x
+
+    def warm_start_guess(self, b, base=None):
+        """The network's guess base + gamma N(b - A base) (base None: gamma N(b))."""
+        x = self.zeros_cell()
+        self._check(self.lib.ds_warm_start_guess(self.h, _ptr(b), _ptr(base), _ptr(x)), "ds_warm_start_guess")
+        return x
+
+    def warm_start_pair(self, error, residual=None, energy=None):
+        """A training pair of the network: r = A e and <e, A e> (device float64 scalar tensor)."""
+        r = self.zeros_cell() if residual is None else residual
+        en = self._zeros((1,), dtype=self.torch.float64) if energy is None else energy
+        self._check(self.lib.ds_warm_start_pair(self.h, _ptr(error), _ptr(r), _ptr(en)), "ds_warm_start_pair")
+        return r, en
 
     def profile_enable(self, sample_every: int):
         """Bracket every `sample_every`-th launch of the PCG iteration kernels with HIP events."""

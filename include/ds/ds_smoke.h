@@ -100,12 +100,20 @@ enum { DS_CG_SINGLE_REDUCTION = 0, DS_CG_TWO_REDUCTION = 1, DS_CG_SINGLE_REDUCTI
  * restarts the sequence, and so does every trajectory entry point. */
 /* DS_WARM_START_PROJECTION: each forward and each adjoint solve starts from the A-norm-best
  * combination of the last 8 solutions of its sequence (Galerkin projection onto their span). */
+/* DS_WARM_START_CNN: the warm-start network corrects the sequence guess of DS_WARM_START_EXTRAPOLATE
+ * (forward solves: the extrapolation; adjoint solves: the previous adjoint solution): with x_b that
+ * guess, the network maps the residual r = b - A x_b to a direction d and the solve starts from
+ * x_b + gamma d, gamma = <d, r> / <d, A d> (the A-norm-best step along d), never worse in the
+ * A-norm than x_b alone. A solve without a base guess (the first of a sequence, the per-operator
+ * projection entry points) starts from the network's direction for b alone, at its best multiple.
+ * DS_WARM_START_CNN_ALONE: every solve starts from the network's direction for b alone. */
 enum {
     DS_WARM_START_NONE = 0,
     DS_WARM_START_PREVIOUS = 1,
     DS_WARM_START_CNN = 2,
     DS_WARM_START_EXTRAPOLATE = 3,
-    DS_WARM_START_PROJECTION = 4
+    DS_WARM_START_PROJECTION = 4,
+    DS_WARM_START_CNN_ALONE = 5
 };
 
 /* smoke::Configuration (data.ixx:72-83) plus the solver fields this build adds. */
@@ -249,6 +257,10 @@ ds_status ds_context_domain_info(ds_context* context, ds_domain_info* info);
  * to host arrays of cell_count bytes. Either pointer may be NULL. Synchronises. */
 ds_status ds_context_domain_masks(ds_context* context, uint8_t* mask, uint8_t* code);
 /* Resource::allocate / release / copy_* / zero / synchronize (cuda.cpp:39-66). */
+/* The last solution (hi + lo, rounded to FP32; DEVICE cell field) of the forward pressure
+ * solves (which = 0) or of the adjoint solves of ds_vjp_step (which = 1): the training pairs of
+ * the warm-start network are formed from them. No counterpart in the reference. */
+ds_status ds_context_last_solution(ds_context* context, uint32_t which, float* out);
 ds_status ds_allocate(ds_context* context, size_t bytes, void** out);
 ds_status ds_release(ds_context* context, void* pointer);
 ds_status ds_copy_from_host(ds_context* context, void* device, const void* host, size_t bytes);
@@ -352,15 +364,72 @@ ds_status ds_apply_pressure_matrix(ds_context* c, const float* x, float* y);
  * starts every solve from p = 0, operators.cu:1005) ------------------------------------------ */
 /* Number of FP32 parameters of the network (a 3-level 3-D U-Net, 16/32/64 channels). */
 uint64_t ds_warm_start_parameter_count(void);
-/* Loads the network's parameters (PyTorch state_dict order of ds_smoke/warm_start.py's UNet:
- * per layer the weight, then the bias) and the scale of its output. Needs nx % 64 == 0,
- * ny % 16 == 0, nz % 4 == 0. With DS_WARM_START_CNN every forward solve without an explicit
- * guess starts from the network's guess, scaled to its A-norm-best multiple (rejected, x0 = 0, when
- * non-finite); the adjoint solves of ds_vjp_step start from the previous adjoint solution, since
- * the network is trained on forward pairs. */
+/* Loads the network's parameters (FP32, PyTorch state_dict order of the 3-level U-Net: per layer
+ * the weight, then the bias; ds_unet_trainer_read writes them in this order). Needs nx % 64 == 0,
+ * ny % 16 == 0, nz % 4 == 0. output_scale (> 0) is kept for compatibility: every guess is applied
+ * at its A-norm-best multiple, so a scale of the network's output cancels. A non-finite guess is
+ * rejected (x0 = 0). */
 ds_status ds_warm_start_load(ds_context* c, const float* params, uint64_t count, double output_scale);
-/* The network's guess x for A x = b (device cell fields; b is 0 off the unknowns). */
+/* The network's guess x for A x = b from b alone (device cell fields; b is 0 off the unknowns). */
 ds_status ds_warm_start_infer(ds_context* c, const float* b, float* x);
+/* The network's correction of a base guess: x = base + gamma N(b - A base) at the A-norm-best
+ * gamma; base = NULL is ds_warm_start_infer. */
+ds_status ds_warm_start_guess(ds_context* c, const float* b, const float* base, float* x);
+
+/* ---- training of the warm-start network on the GPU (no counterpart in the reference) ---------
+ * A training sample is an error e on the unknowns (a solution, or a solution minus a base guess);
+ * its residual r = A e is the network's input (as r / rms(r), with the unknown indicator) and the
+ * network's output g is applied at the A-norm-best step gamma = <g, r> / <g, A g>. Losses: 0 =
+ * energy, 1 - <g, r>^2 / (<g, A g> <e, r>); 1 = L2, ||gamma g - e||^2 / ||e||^2. */
+/* r = A e (0 off the unknowns) and energy = <e, A e> (DEVICE double) for the context's matrix. */
+ds_status ds_warm_start_pair(ds_context* c, const float* error, float* residual, double* energy);
+typedef struct ds_unet_trainer ds_unet_trainer;
+/* A trainer over batches of `batch` samples on the context's grid (nx % 64 == 0, ny % 16 == 0,
+ * nz % 4 == 0) and matrix, on the context's stream, starting from params (HOST, FP32,
+ * ds_warm_start_parameter_count values). Adam state starts at zero. */
+ds_status ds_unet_trainer_create(ds_context* c, uint32_t batch, const float* params, uint64_t count, ds_unet_trainer** out);
+ds_status ds_unet_trainer_destroy(ds_unet_trainer* t);
+/* One step on `batch` errors (DEVICE FP32, batch x cells, sample-major, 0 off the unknowns):
+ * residuals, forward pass, loss of loss_kind (DEVICE, batch doubles, per sample), gradient of the
+ * batch mean by the backward pass; with learning_rate > 0 then one Adam step (beta1 0.9, beta2
+ * 0.999, eps 1e-8) and the repacking of the BF16 weights. */
+ds_status ds_unet_trainer_step(ds_unet_trainer* t, const float* error, uint32_t loss_kind, float learning_rate, double* loss);
+/* Copies (DEVICE FP32) what = 0: the parameters, 1: the gradient of the last step (both
+ * ds_warm_start_parameter_count values), 2: the network output g of the last step (batch x cells). */
+ds_status ds_unet_trainer_read(ds_unet_trainer* t, uint32_t what, float* out);
+
+/* The network's operators on BF16 channels-last tensors (DEVICE; `batch` samples stacked along z;
+ * FP32 weights in PyTorch layout), for the network's layer shapes only: 3x3x3 convolutions
+ * (cin1 + cin2 -> cout) 2->16, 16->16, 16->32, 32->32, 32->64, 64->64, 32+32->32, 16+16->16 (in2,
+ * cin2: the concatenated skip connection); 2x2x2 stride-2 transposed convolutions 64->32, 32->16
+ * (grid = the coarse input grid); 2x2x2 max pooling of 16 or 32 channels (grid = the pooled grid).
+ * nx % 16 == 0 (and ny % 4 == 0 for the 3x3x3 convolution). The 2-channel input is stored with 4
+ * channels (the last two 0). Results are BF16 (round to nearest even) unless stated. Each call
+ * allocates its packing and partial-sum buffers and waits for its stream. */
+ds_status ds_unet_conv3_forward(uint32_t batch, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t cin1, uint32_t cin2, uint32_t cout,
+                                uint32_t relu, const uint16_t* in1, const uint16_t* in2, const float* weight, const float* bias,
+                                uint16_t* out, void* stream);
+/* Input gradient from the output gradient dz: cin2 = 0: din1 = (input_relu > 0 or no mask when
+ * NULL) * dgrad; cin2 > 0: din1 = the first cin1 channels (BF16), din2 = the last cin2 (FP32). */
+ds_status ds_unet_conv3_dgrad(uint32_t batch, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t cin1, uint32_t cin2, uint32_t cout,
+                              const uint16_t* dz, const float* weight, const uint16_t* input_relu, uint16_t* din1, float* din2,
+                              void* stream);
+/* Weight (PyTorch layout) and bias gradients (FP32, overwritten). */
+ds_status ds_unet_conv3_wgrad(uint32_t batch, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t cin1, uint32_t cin2, uint32_t cout,
+                              const uint16_t* in1, const uint16_t* in2, const uint16_t* dz, float* dweight, float* dbias, void* stream);
+ds_status ds_unet_tconv2_forward(uint32_t batch, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t cin, uint32_t cout, const uint16_t* in,
+                                 const float* weight, const float* bias, uint16_t* out, void* stream);
+/* din = (input_relu > 0) * dgrad. */
+ds_status ds_unet_tconv2_dgrad(uint32_t batch, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t cin, uint32_t cout, const uint16_t* dout,
+                               const float* weight, const uint16_t* input_relu, uint16_t* din, void* stream);
+ds_status ds_unet_tconv2_wgrad(uint32_t batch, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t cin, uint32_t cout, const uint16_t* in,
+                               const uint16_t* dout, float* dweight, float* dbias, void* stream);
+ds_status ds_unet_maxpool_forward(uint32_t batch, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t c, const uint16_t* in, uint16_t* out,
+                                  void* stream);
+/* din = (in > 0) * (skip_gradient (FP32, NULL: 0) + dout routed to the first maximum of each
+ * 2x2x2 group in the order dx + 2 dy + 4 dz). */
+ds_status ds_unet_maxpool_backward(uint32_t batch, uint32_t nx, uint32_t ny, uint32_t nz, uint32_t c, const uint16_t* in,
+                                   const uint16_t* dout, const float* skip_gradient, uint16_t* din, void* stream);
 
 /* ---- kernel timing (no counterpart in the reference, which times whole steps with
  * std::chrono, examples/smoke/forward/simulation.cpp:91,107-119) -------------------------- */

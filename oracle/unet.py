@@ -36,10 +36,11 @@ def lib():
         L.unet_parameter_count.restype = C.c_int64
         L.unet_network.argtypes = [I, I, I, I, DP, DP, I, DP, DP, DP]
         L.unet_energy_loss.argtypes = [I, I, I, I, C.c_void_p, C.c_uint, DP, DP, DP, DP, DP]
+        L.unet_l2_loss.argtypes = [I, I, I, I, C.c_void_p, C.c_uint, DP, DP, DP, DP]
         L.unet_adam.argtypes = [C.c_int64, DP, DP, DP, DP, I, C.c_double, C.c_double, C.c_double, C.c_double]
         for f in ("unet_conv3_forward", "unet_conv3_dgrad", "unet_conv3_wgrad", "unet_tconv2_forward", "unet_tconv2_dgrad",
                   "unet_tconv2_wgrad", "unet_maxpool_forward", "unet_maxpool_backward", "unet_network", "unet_energy_loss",
-                  "unet_adam"):
+                  "unet_l2_loss", "unet_adam"):
             getattr(L, f).restype = None
         _lib = L
     return _lib
@@ -147,6 +148,17 @@ def energy_loss(code, g, r, e, periodic=0):
     loss = np.zeros(B)
     dg = np.zeros((B, nz, ny, nx))
     lib().unet_energy_loss(B, nx, ny, nz, code.ctypes.data_as(C.c_void_p), periodic, _p(_f(g)), _p(_f(r)), _p(_f(e)), _p(loss), _p(dg))
+    return loss, dg
+
+
+def l2_loss(code, g, e, periodic=0):
+    """Per-sample loss ||gamma g - e||^2 / ||e||^2 (gamma = <g, A e> / <g, A g>) and its gradient with
+    respect to g. code: (nz, ny, nx) uint8; g, e: (B, nz, ny, nx)."""
+    B, nz, ny, nx = g.shape
+    code = np.ascontiguousarray(code, dtype=np.uint8)
+    loss = np.zeros(B)
+    dg = np.zeros((B, nz, ny, nx))
+    lib().unet_l2_loss(B, nx, ny, nz, code.ctypes.data_as(C.c_void_p), periodic, _p(_f(g)), _p(_f(e)), _p(loss), _p(dg))
     return loss, dg
 
 

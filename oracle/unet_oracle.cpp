@@ -498,6 +498,40 @@ void unet_energy_loss(int B, int nx, int ny, int nz, const std::uint8_t* code, u
     }
 }
 
+// L2 loss of B samples and its gradient with respect to g, per sample: with r = A e,
+// gamma = <g, r> / <g, A g> (the step the solver applies) and f = gamma g - e:
+// loss[b] = ||f||^2 / ||e||^2, dL/dg = (2 <f, g> (r - 2 gamma A g) / <g, A g> + 2 gamma f) / ||e||^2.
+void unet_l2_loss(int B, int nx, int ny, int nz, const std::uint8_t* code, unsigned periodic, const double* g, const double* e,
+                  double* loss, double* dg) {
+    const Dim d{nx, ny, nz};
+    const i64 n = d.n();
+    std::vector<double> ag(n), r(n);
+    for (int b = 0; b < B; ++b) {
+        const double* gb = g + (i64)b * n;
+        const double* eb = e + (i64)b * n;
+        apply_matrix(d, code, periodic, gb, ag.data());
+        apply_matrix(d, code, periodic, eb, r.data());
+        double q = 0.0, D = 0.0, GG = 0.0, P = 0.0, E2 = 0.0;
+        for (i64 i = 0; i < n; ++i) {
+            if (!code[i]) continue;
+            q += gb[i] * r[i];
+            D += gb[i] * ag[i];
+            GG += gb[i] * gb[i];
+            P += gb[i] * eb[i];
+            E2 += eb[i] * eb[i];
+        }
+        const double gamma = q / D;
+        loss[b] = (gamma * gamma * GG - 2.0 * gamma * P + E2) / E2;
+        if (dg) {
+            const double fg = gamma * GG - P;
+            for (i64 i = 0; i < n; ++i) {
+                const double f = gamma * gb[i] - eb[i];
+                dg[(i64)b * n + i] = code[i] ? (2.0 * fg * (r[i] - 2.0 * gamma * ag[i]) / D + 2.0 * gamma * f) / E2 : 0.0;
+            }
+        }
+    }
+}
+
 // Adam (Kingma & Ba, bias-corrected): m = b1 m + (1 - b1) g, v = b2 v + (1 - b2) g^2,
 // p -= lr (m / (1 - b1^t)) / (sqrt(v / (1 - b2^t)) + eps); step t counts from 1.
 void unet_adam(std::int64_t n, double* p, const double* g, double* m, double* v, int t, double lr, double b1, double b2, double eps) {

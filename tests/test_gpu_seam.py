@@ -37,8 +37,10 @@ def seam(request):
     from ds_smoke.solver import Context
 
     sc = seam_scene(request.param)
-    ctx = Context(sc.config(tolerance=1e-6), abi.MODE_DIFFERENTIABLE)
-    orc = O.Oracle(sc, tolerance=1e-10)
+    # solves to 1e-8 unless a test asks for 1e-6: the 136-cell box is ill-conditioned enough that a
+    # 1e-6 residual leaves solution errors of a few 1e-5
+    ctx = Context(sc.config(tolerance=1e-8), abi.MODE_DIFFERENTIABLE)
+    orc = O.Oracle(sc, tolerance=1e-11)
     return request.param, sc, ctx, orc
 
 
@@ -49,16 +51,25 @@ def test_pcg_across_tiles(seam):
     np.testing.assert_array_equal(code, orc.masks()[1])
     rng = np.random.default_rng(136)
     b = np.where(code > 0, rng.standard_normal(code.shape), 0.0).astype(np.float32)
-    x, info, st = ctx.pcg_solve(ctx.to_device(b))
+    b64 = b.astype(np.float64)
+    ref, _ = orc.pcg(b64, tol=1e-12)
+    # At 1e-6 the solution's error is bounded only through the condition number (about 2e4 for a
+    # 136-cell-long box), so the solve to 1e-8 is the one that pins the seam.
+    x, info, st = ctx.pcg_solve(ctx.to_device(b), tolerance=1e-6)
     assert st == 0 and info.converged
-    ref, _ = orc.pcg(b.astype(np.float64), tol=1e-12)
-    err = rel(ctx.to_host(x), ref)
+    err6 = rel(ctx.to_host(x), ref)
+    x8, info8, st8 = ctx.pcg_solve(ctx.to_device(b), tolerance=1e-8)
+    assert st8 == 0 and info8.converged
+    err8 = rel(ctx.to_host(x8), ref)
     # the matrix-vector product itself, cell by cell, against the oracle's
     xs = rng.standard_normal(code.shape).astype(np.float32) * (code > 0)
     ax = ctx.to_host(ctx.apply_matrix(ctx.to_device(xs))).astype(np.float64)
     ax_ref = orc.apply_matrix(xs.astype(np.float64))
-    print(f"  {info.iterations} iterations; solution {err:.2e}; A x max |diff| {np.abs(ax - ax_ref).max():.2e}")
-    assert err < 5e-6, err
+    print(f"  1e-6: {info.iterations} iterations, solution {err6:.2e}; "
+          f"1e-8: {info8.iterations} iterations, solution {err8:.2e}; A x max |diff| {np.abs(ax - ax_ref).max():.2e}")
+    # measured: 4.4e-5 / 3.7e-5 at 1e-6 (the condition number's share), 3.3e-8 / 2.5e-8 at 1e-8
+    assert err6 < 3e-4, err6
+    assert err8 < 3e-7, err8
     np.testing.assert_allclose(ax, ax_ref, rtol=0, atol=1e-5)
 
 
@@ -82,6 +93,16 @@ def test_step_across_tiles(seam):
     gc = ctx.to_host(cache)
     p = gc["pressure"][0].astype(np.float64) + gc["pressure"][1]
     ep = rel(p, ref_cache["pressure"])
-    print(f"  step periodic_x={periodic}: state {max(errs):.2e}, pressure {ep:.2e}")
+    # the projection alone, from the GPU's own advected velocity: the seam's share of the step
+    adv = [rel(gc["advected_velocity"][a], ref_cache["advected_velocity"][a]) for a in range(3)]
+    _, p_own, _, _ = orc.projection_forward(tuple(v.astype(np.float64) for v in gc["advected_velocity"]))
+    e_own = rel(p, p_own)
+    print(f"  step periodic_x={periodic}: state {max(errs):.2e}, advected velocity {max(adv):.2e}, "
+          f"pressure {ep:.2e} (from the GPU's advected velocity {e_own:.2e}); solve: {info.iterations} iterations, "
+          f"final residual {info.final_residual / max(info.rhs_norm, 1e-300):.2e} relative")
     assert max(errs) < 1e-5, errs
-    assert ep < 1e-5, ep
+    assert max(adv) < 5e-5, adv
+    assert e_own < 3e-7, e_own
+    # FP32 against FP64 advection beside the solids moves the right-hand side, which the solve
+    # spreads: the same bound as the single-tile step tests
+    assert ep < 1e-4, ep

@@ -201,6 +201,34 @@ def test_energy_loss_zero_for_exact_error_and_gradient_by_differences(rng):
         np.testing.assert_allclose(dg.reshape(2, -1)[:, i], fd, rtol=1e-6, atol=1e-10)
 
 
+def test_l2_loss_zero_for_any_multiple_of_the_error_and_gradient_by_differences(rng):
+    """||gamma g - e||^2 / ||e||^2 with gamma = <g, A e> / <g, A g>: 0 for g = c e at any c (the step
+    undoes the scale), the closed form with a dense matrix, and the gradient by central differences."""
+    code = closed_code(3, 4, 5)
+    A = dense_matrix(code)
+    unk = code.reshape(-1) > 0
+    e = np.zeros((2, code.size))
+    e[:, unk] = rng.standard_normal((2, unk.sum()))
+    shp = (2,) + code.shape
+    for c in (0.25, -3.0):
+        loss, _ = U.l2_loss(code, c * e.reshape(shp), e.reshape(shp))
+        np.testing.assert_allclose(loss, 0.0, atol=1e-12)
+    g = np.zeros_like(e)
+    g[:, unk] = rng.standard_normal((2, unk.sum()))
+    loss, dg = U.l2_loss(code, g.reshape(shp), e.reshape(shp))
+    for b in range(2):
+        gamma = (g[b] @ A @ e[b]) / (g[b] @ A @ g[b])
+        np.testing.assert_allclose(loss[b], np.sum((gamma * g[b] - e[b]) ** 2) / np.sum(e[b] ** 2), rtol=1e-12)
+    eps = 1e-6
+    for i in np.flatnonzero(unk)[:6]:
+        gp, gm = g.copy(), g.copy()
+        gp[:, i] += eps
+        gm[:, i] -= eps
+        fd = (U.l2_loss(code, gp.reshape(shp), e.reshape(shp))[0] - U.l2_loss(code, gm.reshape(shp), e.reshape(shp))[0]) / (2 * eps)
+        np.testing.assert_allclose(dg.reshape(2, -1)[:, i], fd, rtol=1e-6, atol=1e-10)
+    assert np.all(dg.reshape(2, -1)[:, ~unk] == 0.0)
+
+
 def test_adam_first_step_is_lr_times_sign():
     p = np.array([1.0, -2.0, 3.0, 0.5])
     g = np.array([0.3, -4.0, 1e-3, -2e-2])
