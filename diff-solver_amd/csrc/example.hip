@@ -7,6 +7,8 @@
 // with FP64 atomics, so its sums vary from run to run in the last bits; here every block leaves
 // six partials in a fixed slot and one block finishes them in a fixed order, so the result is
 // the same on every run.
+#include <mutex>
+
 #include "kernels.hpp"
 
 namespace ds {
@@ -94,6 +96,7 @@ __device__ __forceinline__ void block_metrics(double (&v)[kMetrics], double* lds
 // fluid cells only (mask == 0); the divergence is the FP32 sum of the six face differences in
 // the reference's order, scaled by 1/h in FP64; the cell velocity is the FP64 half of each FP32
 // face-pair sum. Block b leaves its six values at partials[k * gridDim.x + b].
+// A null mask means no solid cells.
 __global__ __launch_bounds__(256) void metrics_partial_kernel(Geo g, const uint8_t* __restrict__ mask, const float* __restrict__ rho,
                                                               const float* __restrict__ T, CF3 pre, CF3 post,
                                                               double* __restrict__ partials) {
@@ -102,7 +105,7 @@ __global__ __launch_bounds__(256) void metrics_partial_kernel(Geo g, const uint8
     const double inv_h = 1.0 / (double)g.h;
     const uint32_t plane = (uint32_t)g.nx * (uint32_t)g.ny;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x) {
-        if (mask[i] != 0) continue;
+        if (mask && mask[i] != 0) continue;
         const uint32_t yz = fdiv(i, g.cx);
         const uint32_t x = i - yz * (uint32_t)g.nx;
         const uint32_t z = fdiv(yz, g.cy);
@@ -143,6 +146,31 @@ __global__ __launch_bounds__(256) void metrics_finalize_kernel(const double* __r
     if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < kMetrics; ++k) out[k] = v[k];
+}
+
+// Partials of ds_example_reduce_metrics_static, which allocates nothing: one slot per block and
+// metric (kMetrics * kMetricBlocks FP64 = 24 KiB per GPU, held until the process ends). The launch
+// always uses kMetricBlocks blocks, so every grid the kernels can index fits in it. Host code
+// never passes this name to a kernel: it stands for a host-side shadow there; the device address
+// comes from hipGetSymbolAddress, per device.
+__device__ __align__(256) double scratch[kMetrics * kMetricBlocks];
+
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch[kScratchDevices] = {};
+
+// The device address of `scratch` on the current device, looked up once per device.
+double* metrics_scratch() {
+    int dev = 0;
+    DS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kScratchDevices) throw Failure{DS_ERROR_UNSUPPORTED, "device ordinal beyond the scratch table"};
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        DS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = static_cast<double*>(p);
+    }
+    return g_scratch[dev];
 }
 
 }  // namespace
@@ -197,6 +225,47 @@ ds_status ds_example_reduce_metrics(ds_context* c, const float* density, const f
             CF3{{post_projection_velocity[0], post_projection_velocity[1], post_projection_velocity[2]}}, c->metric_partials);
         check_launch("metrics_partial_kernel");
         metrics_finalize_kernel<<<1, kBlock, 0, c->stream>>>(c->metric_partials, kMetricBlocks, metrics);
+        check_launch("metrics_finalize_kernel");
+    });
+}
+
+ds_status ds_example_reduce_metrics_static(const ds_config* config, const uint8_t* solid_mask, const float* density,
+                                           const float* temperature, const float* const pre_projection_velocity[3],
+                                           const float* const post_projection_velocity[3], double* metrics, void* stream) {
+    if (!config || !density || !temperature || !pre_projection_velocity || !post_projection_velocity || !metrics ||
+        !pre_projection_velocity[0] || !pre_projection_velocity[1] || !pre_projection_velocity[2] || !post_projection_velocity[0] ||
+        !post_projection_velocity[1] || !post_projection_velocity[2]) {
+        set_error("ds_example_reduce_metrics_static: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    const uint64_t nx = config->resolution[0], ny = config->resolution[1], nz = config->resolution[2];
+    if (!nx || !ny || !nz || !(config->cell_size > 0.0f)) {
+        set_error("ds_example_reduce_metrics_static: resolution must be positive and cell_size > 0");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    // the kernels index cells and faces in 32 bits
+    if ((nx + 1) * (ny + 1) * (nz + 1) >= (1ull << 32)) {
+        set_error("ds_example_reduce_metrics_static: grid too large (cell and face indices must fit in 32 bits)");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] {
+        double* partials = metrics_scratch();
+        Geo g{};
+        g.nx = (int)nx;
+        g.ny = (int)ny;
+        g.nz = (int)nz;
+        g.h = config->cell_size;
+        g.inv_h = 1.0f / config->cell_size;
+        g.dt = config->time_step;
+        g.n = (uint32_t)(nx * ny * nz);
+        g.cx = make_fastdiv((uint32_t)nx);
+        g.cy = make_fastdiv((uint32_t)ny);
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        metrics_partial_kernel<<<kMetricBlocks, kBlock, 0, s>>>(
+            g, solid_mask, density, temperature, CF3{{pre_projection_velocity[0], pre_projection_velocity[1], pre_projection_velocity[2]}},
+            CF3{{post_projection_velocity[0], post_projection_velocity[1], post_projection_velocity[2]}}, partials);
+        check_launch("metrics_partial_kernel");
+        metrics_finalize_kernel<<<1, kBlock, 0, s>>>(partials, kMetricBlocks, metrics);
         check_launch("metrics_finalize_kernel");
     });
 }

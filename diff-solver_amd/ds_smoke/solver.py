@@ -119,6 +119,7 @@ def load_library(path: str | None = None):
                                           C.POINTER(abi.ParameterAdjoint), C.POINTER(abi.TrajectoryInfo)]),
         "ds_example_write_control": (C.c_int32, [P, C.c_uint64, C.POINTER(abi.JetControl), P, P, C.POINTER(P)]),
         "ds_example_reduce_metrics": (C.c_int32, [P, P, P, C.POINTER(P), C.POINTER(P), P]),
+        "ds_example_reduce_metrics_static": (C.c_int32, [C.POINTER(abi.Config), P, P, P, C.POINTER(P), C.POINTER(P), P, P]),
         "ds_warm_start_parameter_count": (C.c_uint64, []),
         "ds_warm_start_load": (C.c_int32, [P, P, C.c_uint64, C.c_double]),
         "ds_warm_start_infer": (C.c_int32, [P, P, P]),

@@ -549,6 +549,20 @@ ds_status ds_example_reduce_metrics(ds_context* c, const float* density, const f
                                     const float* const pre_projection_velocity[3],
                                     const float* const post_projection_velocity[3], double* metrics);
 
+/* ds_example_reduce_metrics without a context, for callers that cannot allocate (code under HIP
+ * graph capture, serving loops, harnesses that forbid hipMalloc). No counterpart in the reference.
+ * It allocates nothing: its per-block partials live in a fixed device array of the library
+ * (6 x 512 FP64 = 24 KiB per GPU, held until the process ends), which covers every grid whose
+ * cell and face indices fit in 32 bits; larger grids are rejected (DS_ERROR_INVALID_ARGUMENT).
+ * Only config->resolution and config->cell_size are read; solid_mask (DEVICE, one byte per cell,
+ * nonzero = solid, as ds_context_domain_masks returns) may be NULL for no solids; stream may be
+ * NULL (the null stream). The result equals ds_example_reduce_metrics's bit for bit. Every call
+ * on one GPU shares that array: calls that overlap on one GPU (other streams, other threads)
+ * race and must be serialised by the caller. */
+ds_status ds_example_reduce_metrics_static(const ds_config* config, const uint8_t* solid_mask, const float* density,
+                                           const float* temperature, const float* const pre_projection_velocity[3],
+                                           const float* const post_projection_velocity[3], double* metrics, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
