@@ -105,10 +105,11 @@ UNET_MACS_PER_VOXEL = (27 * (2 * 16 + 16 * 16 + 32 * 16 + 16 * 16) + 32 * 16 + 1
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def unet_timing(ctx, sc, repeats=20):
-    """MFMA roofline of the network's inference at the workload's grid: algorithmic FLOPs
-    (2 x UNET_MACS_PER_VOXEL per voxel) / the mean time of `repeats` inferences (HIP events on the
-    context's stream, after one warm-up)."""
+def unet_timing(ctx, sc, levels, repeats=20):
+    """MFMA roofline of the network's guess at the workload's grid: algorithmic FLOPs
+    (2 x UNET_MACS_PER_VOXEL per voxel of the grid the network runs on, the workload's grid
+    coarsened `levels` times) / the mean time of `repeats` whole guesses (restriction, network,
+    prolongation and step length; HIP events on the context's stream, after one warm-up)."""
     import numpy as np
     import torch
 
@@ -124,9 +125,10 @@ def unet_timing(ctx, sc, repeats=20):
     stop.record(ctx.stream)
     stop.synchronize()
     ms = start.elapsed_time(stop) / repeats
-    flops = 2.0 * UNET_MACS_PER_VOXEL * int(np.prod(sc.resolution))
+    net_grid = [int(n) >> levels for n in sc.resolution]
+    flops = 2.0 * UNET_MACS_PER_VOXEL * int(np.prod(net_grid))
     tf = flops / (ms * 1e-3) / 1e12
-    return {"bound": "mfma", "kernel": "warm-start network inference (13 layers)", "achieved": round(tf, 2),
+    return {"bound": "mfma", "kernel": "warm-start network guess (13 layers)", "network_grid": net_grid, "achieved": round(tf, 2),
             "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(tf / MFMA_BF16_PEAK_TFLOPS, 4),
             "flops_per_inference": flops, "ms_per_inference": round(ms, 4), "dtype": "bf16 (fp32 accumulation)"}
 
@@ -193,9 +195,8 @@ def main():
         if mode in ("cnn", "cnn_alone"):
             from ds_smoke import warm_start
 
-            flat, output_scale, _ = warm_start.load(args.weights)
-            ctx.warm_start_load(flat, output_scale)
-            unet = unet_timing(ctx, sc)
+            meta = warm_start.load_into(ctx, args.weights)
+            unet = unet_timing(ctx, sc, warm_start.levels_for(meta, sc.resolution))
         s0 = ctx.state_from_host(scenes.initial_state(sc))
         ctl = ctx.control_from_host(scenes.control(sc))
         params = ctx.allocate_parameters(sc.params)

@@ -308,6 +308,8 @@ void launch_apply_matrix(ds_context* c, const float* x, float* y);
 // Warm-start network (unet.hip).
 uint64_t unet_parameter_count();
 void unet_load(ds_context* c, const float* params, uint64_t count, double output_scale);
+// Runs the network on the solve's grid coarsened 2^levels times (0, 1, 2).
+void unet_configure(ds_context* c, uint32_t levels);
 // The network's guess for A x = b (device b), corrected from the base guess xb when not null.
 const float* unet_infer(ds_context* c, const float* b, const float* xb);
 bool unet_loaded(const ds_context* c);

@@ -104,7 +104,7 @@ __global__ __launch_bounds__(256) void extrapolate_kernel(uint32_t n, int order,
 std::pair<const float*, const float*> sequence_guess(ds_context* c, float* hi, float* lo, bool valid, float* prev, int& prev_count,
                                                      bool previous_under_network = false) {
     uint32_t mode = c->config.warm_start;
-    // the network corrects the extrapolated forward guess and the previous adjoint solution
+    // the network corrects the extrapolated forward guess; adjoint solves keep the previous solution
     if (mode == DS_WARM_START_CNN) mode = previous_under_network ? DS_WARM_START_PREVIOUS : DS_WARM_START_EXTRAPOLATE;
     if (!valid || (mode != DS_WARM_START_PREVIOUS && mode != DS_WARM_START_EXTRAPOLATE)) return {nullptr, nullptr};
     if (mode == DS_WARM_START_EXTRAPOLATE && prev) {
@@ -213,11 +213,13 @@ ds_solve_info vjp_step(ds_context* c, const ds_parameters* p, const ds_state* st
     const float* ex[3]{c->projected_extra[0], c->projected_extra[1], c->projected_extra[2]};
     // adjoint solves start from the previous adjoint solution under both sequence modes (on the
     // 128^3 plume the extrapolated adjoint guess took more iterations, 606 per solve, than the
-    // previous solution, 566); under the network, the network corrects that solution
+    // previous solution, 566). Under DS_WARM_START_CNN the network does not correct it: on the same
+    // trajectory the corrected guess took 575 iterations per adjoint solve against the previous
+    // solution's 565, and each correction costs an inference (0.55 ms at 128^3 through the 64^3 grid)
     int no_prev = 0;
     const auto lg = sequence_guess(c, c->lambda_hi, c->lambda_lo, c->lambda_valid, nullptr, no_prev, true);
     const ds_solve_info si = pcg_solve_kind(c, 2, na, ex, c->lambda_hi, c->lambda_lo, lg.first, lg.second, c->tolerance, c->max_iterations,
-                                            c->config.warm_start == DS_WARM_START_CNN);
+                                            false);
     c->lambda_valid = true;
     launch_projection_vjp_faces(c, na, ex, c->lambda_hi, c->lambda_lo, F3{{c->raw_adj[0], c->raw_adj[1], c->raw_adj[2]}}, true);
     launch_advect_velocity_vjp(c, CF3{{cache->forced_velocity[0], cache->forced_velocity[1], cache->forced_velocity[2]}},

@@ -775,6 +775,14 @@ ds_status ds_warm_start_load(ds_context* c, const float* params, uint64_t count,
     return guard([&] { unet_load(c, params, count, output_scale); });
 }
 
+ds_status ds_warm_start_configure(ds_context* c, uint32_t levels) {
+    if (!c) {
+        set_error("ds_warm_start_configure: null context");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] { unet_configure(c, levels); });
+}
+
 ds_status ds_warm_start_infer(ds_context* c, const float* b, float* x) {
     if (!c || !b || !x) {
         set_error("ds_warm_start_infer: null argument");

@@ -8,6 +8,13 @@
 // x0 = x_b + gamma d with gamma = <d, r> / <d, A d>, the step along d that minimises the A-norm
 // error (the norm CG minimises), so x0 is never further from x* in that norm than x_b. The network
 // is trained for exactly this use: its loss is the error of x_b + gamma d (unet_train.hip).
+//
+// With `levels` L > 0 the network runs on the grid coarsened 2^L times per axis: r is restricted
+// (the sum over each coarse cell's children; a coarse cell is an unknown when any child is), the
+// network's coarse output is prolonged back by cell-centred trilinear interpolation (clamped at the
+// walls) and masked to the fine unknowns, and gamma is formed on the fine grid. A network trained
+// on 64^3 problems then sees 128^3 (L = 1) or 256^3 (L = 2) problems at the size it was trained
+// on, its receptive field covers the same share of the domain, and it costs 8^L times less.
 #include <cmath>
 #include <vector>
 
@@ -46,6 +53,73 @@ __global__ __launch_bounds__(256) void base_residual_kernel(Geo geo, uint32_t pe
                 if (code[j]) s -= (double)xb[j];
             }
         r[i] = (float)((double)b[i] - s);
+    }
+}
+
+// rc[I] = sum of r over the f^3 children of coarse cell I (f = 2^levels).
+__global__ __launch_bounds__(256) void restrict_kernel(Geo g, int f, Dims dc, const float* __restrict__ r, float* __restrict__ rc) {
+    const uint32_t nc = (uint32_t)dc.nx * dc.ny * dc.nz;
+    for (uint32_t I = blockIdx.x * blockDim.x + threadIdx.x; I < nc; I += gridDim.x * blockDim.x) {
+        const int X = (int)(I % (uint32_t)dc.nx), Y = (int)((I / (uint32_t)dc.nx) % (uint32_t)dc.ny), Z = (int)(I / ((uint32_t)dc.nx * dc.ny));
+        float s = 0.0f;
+        for (int k = 0; k < f; ++k)
+            for (int j = 0; j < f; ++j)
+                for (int i = 0; i < f; ++i) s += r[cidx(g, f * X + i, f * Y + j, f * Z + k)];
+        rc[I] = s;
+    }
+}
+
+// ccode[I] = 1 when any child of coarse cell I is an unknown (the network's unknown indicator), and
+// per-block counts of such cells: counts[block].
+__global__ __launch_bounds__(256) void coarse_code_kernel(Geo g, int f, Dims dc, const uint8_t* __restrict__ code, uint8_t* __restrict__ ccode,
+                                                          double* __restrict__ counts) {
+    __shared__ double lds[kMaxWaves];
+    const uint32_t nc = (uint32_t)dc.nx * dc.ny * dc.nz;
+    double v[1]{0.0};
+    for (uint32_t I = blockIdx.x * blockDim.x + threadIdx.x; I < nc; I += gridDim.x * blockDim.x) {
+        const int X = (int)(I % (uint32_t)dc.nx), Y = (int)((I / (uint32_t)dc.nx) % (uint32_t)dc.ny), Z = (int)(I / ((uint32_t)dc.nx * dc.ny));
+        uint8_t any = 0;
+        for (int k = 0; k < f; ++k)
+            for (int j = 0; j < f; ++j)
+                for (int i = 0; i < f; ++i) any |= code[cidx(g, f * X + i, f * Y + j, f * Z + k)] != 0u ? 1u : 0u;
+        ccode[I] = any;
+        v[0] += any;
+    }
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) counts[blockIdx.x] = v[0];
+}
+
+// d[fine] = cell-centred trilinear interpolation of dc (coarse position (x + 1/2) / f - 1/2 per
+// axis, clamped to the coarse grid), 0 off the fine unknowns.
+__device__ __forceinline__ void lin_weights(int x, int f, int n, int& i0, int& i1, float& t) {
+    const float X = ((float)x + 0.5f) / (float)f - 0.5f;
+    const float fl = floorf(X);
+    t = X - fl;
+    const int a = (int)fl;
+    i0 = min(max(a, 0), n - 1);
+    i1 = min(max(a + 1, 0), n - 1);
+}
+__global__ __launch_bounds__(256) void prolong_kernel(Geo g, int f, Dims dc, const uint8_t* __restrict__ code, const float* __restrict__ dcv,
+                                                      float* __restrict__ d) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < g.n; i += gridDim.x * blockDim.x) {
+        if (!code[i]) {
+            d[i] = 0.0f;
+            continue;
+        }
+        int x, y, z;
+        decode_cell(g, i, x, y, z);
+        int x0, x1, y0, y1, z0, z1;
+        float tx, ty, tz;
+        lin_weights(x, f, dc.nx, x0, x1, tx);
+        lin_weights(y, f, dc.ny, y0, y1, ty);
+        lin_weights(z, f, dc.nz, z0, z1, tz);
+        auto at = [&](int X, int Y, int Z) { return dcv[(uint32_t)X + (uint32_t)dc.nx * ((uint32_t)Y + (uint32_t)dc.ny * (uint32_t)Z)]; };
+        const float c00 = at(x0, y0, z0) + tx * (at(x1, y0, z0) - at(x0, y0, z0));
+        const float c10 = at(x0, y1, z0) + tx * (at(x1, y1, z0) - at(x0, y1, z0));
+        const float c01 = at(x0, y0, z1) + tx * (at(x1, y0, z1) - at(x0, y0, z1));
+        const float c11 = at(x0, y1, z1) + tx * (at(x1, y1, z1) - at(x0, y1, z1));
+        const float c0 = c00 + ty * (c10 - c00), c1 = c01 + ty * (c11 - c01);
+        d[i] = c0 + tz * (c1 - c0);
     }
 }
 
@@ -133,7 +207,15 @@ uint64_t unet_parameter_count() {
 }
 
 struct UNet {
-    std::vector<void*> allocations;
+    std::vector<void*> allocations;     // parameters
+    std::vector<void*> activations;     // sized for the network's grid (released when levels change)
+    int levels = 0;                     // 2^levels coarsening of the solve's grid before the network
+    Dims dn{};                          // the network's grid
+    uint8_t* ccode = nullptr;           // the network grid's unknown indicator (levels > 0)
+    uint64_t net_unknowns = 0;          // its unknown count
+    float* rc = nullptr;                // restricted input (levels > 0)
+    float* dc = nullptr;                // network output on the network grid (levels > 0)
+    double* count_partials = nullptr;   // kPcgBlocks + 1
     bf16x4_t* wpk[kUNetLayerCount]{};
     float* bias[kUNetLayerCount]{};
     float* head_w = nullptr;
@@ -151,17 +233,64 @@ struct UNet {
 void unet_release(UNet* u) {
     if (!u) return;
     for (void* p : u->allocations) (void)hipFree(p);
+    for (void* p : u->activations) (void)hipFree(p);
     delete u;
 }
 
 namespace {
 
 template <typename T>
-T* unet_alloc(UNet* u, size_t count) {
+T* unet_alloc(UNet* u, size_t count, bool activation = false) {
     void* p = nullptr;
     DS_HIP(hipMalloc(&p, count * sizeof(T) > 0 ? count * sizeof(T) : 1));
-    u->allocations.push_back(p);
+    (activation ? u->activations : u->allocations).push_back(p);
     return static_cast<T*>(p);
+}
+
+__global__ void count_finish_kernel(const double* __restrict__ partials, uint32_t G, double* __restrict__ out) {
+    __shared__ double lds[kMaxWaves];
+    const double* parts[1]{partials};
+    double red[1];
+    reduce_partials<1>(parts, G, red, lds);
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+// Activation buffers and the coarse unknown indicator for the current levels.
+void unet_prepare(ds_context* c, UNet* u) {
+    if (u->in4) return;
+    const Geo& g = c->geo;
+    const int L = u->levels, f = 1 << L;
+    u->dn = Dims{g.nx >> L, g.ny >> L, g.nz >> L};
+    const size_t n = (size_t)u->dn.nx * u->dn.ny * u->dn.nz, nm = n / 8, nc = n / 64, nf = g.n;
+    u->in4 = unet_alloc<uint16_t>(u, 4 * n, true);
+    u->f1 = unet_alloc<uint16_t>(u, 16 * n, true);
+    u->f2 = unet_alloc<uint16_t>(u, 16 * n, true);
+    u->skip1 = unet_alloc<uint16_t>(u, 16 * n, true);
+    u->m1 = unet_alloc<uint16_t>(u, 32 * nm, true);
+    u->m2 = unet_alloc<uint16_t>(u, 32 * nm, true);
+    u->skip2 = unet_alloc<uint16_t>(u, 32 * nm, true);
+    u->c1 = unet_alloc<uint16_t>(u, 64 * nc, true);
+    u->c2 = unet_alloc<uint16_t>(u, 64 * nc, true);
+    u->resid = unet_alloc<float>(u, nf, true);
+    u->dir = unet_alloc<float>(u, nf, true);
+    u->guess = unet_alloc<float>(u, nf, true);
+    u->norm_partials = unet_alloc<double>(u, (size_t)kPcgBlocks, true);
+    u->step_partials = unet_alloc<double>(u, 2 * (size_t)kPcgBlocks, true);
+    if (L > 0) {
+        u->ccode = unet_alloc<uint8_t>(u, n, true);
+        u->rc = unet_alloc<float>(u, n, true);
+        u->dc = unet_alloc<float>(u, n, true);
+        u->count_partials = unet_alloc<double>(u, (size_t)kPcgBlocks + 1, true);
+        coarse_code_kernel<<<kPcgBlocks, kBlock, 0, c->stream>>>(g, f, u->dn, c->code, u->ccode, u->count_partials);
+        count_finish_kernel<<<1, kBlock, 0, c->stream>>>(u->count_partials, kPcgBlocks, u->count_partials + kPcgBlocks);
+        check_launch("coarse_code_kernel");
+        double cnt = 0.0;
+        DS_HIP(hipMemcpyAsync(&cnt, u->count_partials + kPcgBlocks, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        DS_HIP(hipStreamSynchronize(c->stream));
+        u->net_unknowns = (uint64_t)cnt;
+    } else {
+        u->net_unknowns = c->unknowns;
+    }
 }
 
 // Conv weights -> fragment order: pk[((tap * NCH + ch) * NB + nb) * 64 + l][j] =
@@ -187,15 +316,38 @@ uint32_t wave_blocks(uint64_t waves) { return (uint32_t)((waves + 3) / 4); }
 
 }  // namespace
 
+void unet_configure(ds_context* c, uint32_t levels) {
+    require(levels <= 2, "warm-start levels must be 0, 1 or 2");
+    const Geo& g = c->geo;
+    const int f = 1 << levels;
+    if (g.nx % (64 * f) || g.ny % (16 * f) || g.nz % (4 * f))
+        throw Failure{DS_ERROR_UNSUPPORTED, "the warm-start network needs nx % 64 == 0, ny % 16 == 0 and nz % 4 == 0 on its grid "
+                                            "(the solve's grid coarsened 2^levels times)"};
+    UNet* u = c->unet ? c->unet : new UNet();
+    c->unet = u;
+    if (u->levels == (int)levels) return;
+    DS_HIP(hipStreamSynchronize(c->stream));
+    for (void* p : u->activations) (void)hipFree(p);
+    u->activations.clear();
+    u->in4 = nullptr;
+    u->ccode = nullptr;
+    u->rc = u->dc = nullptr;
+    u->levels = (int)levels;
+    if (u->head_w) unet_prepare(c, u);   // a loaded network: its buffers now, not inside a solve
+}
+
 void unet_load(ds_context* c, const float* params, uint64_t count, double output_scale) {
     (void)output_scale;   // the guess is applied at its A-norm-best multiple, so a scale cancels
     require(params != nullptr, "warm-start parameters are null");
     if (count != unet_parameter_count())
         throw Failure{DS_ERROR_INVALID_ARGUMENT, "warm-start parameter count does not match the network (" +
                                                      std::to_string(unet_parameter_count()) + " expected)"};
-    const Geo& g = c->geo;
-    if (g.nx % 64 || g.ny % 16 || g.nz % 4)
-        throw Failure{DS_ERROR_UNSUPPORTED, "the warm-start network needs nx % 64 == 0, ny % 16 == 0 and nz % 4 == 0"};
+    {
+        const Geo& g = c->geo;
+        const int f = 1 << (c->unet ? c->unet->levels : 0);
+        if (g.nx % (64 * f) || g.ny % (16 * f) || g.nz % (4 * f))
+            throw Failure{DS_ERROR_UNSUPPORTED, "the warm-start network needs nx % 64 == 0, ny % 16 == 0 and nz % 4 == 0 on its grid"};
+    }
     UNet* u = c->unet ? c->unet : new UNet();
     c->unet = u;
     const float* p = params;
@@ -215,27 +367,11 @@ void unet_load(ds_context* c, const float* params, uint64_t count, double output
         DS_HIP(hipMemcpyAsync(u->bias[k], p + nw, L.cout * sizeof(float), hipMemcpyHostToDevice, c->stream));
         p += nw + L.cout;
     }
-    if (!u->in4) {
-        const size_t n = g.n, nm = n / 8, nc = n / 64;
-        u->in4 = unet_alloc<uint16_t>(u, 4 * n);
-        u->f1 = unet_alloc<uint16_t>(u, 16 * n);
-        u->f2 = unet_alloc<uint16_t>(u, 16 * n);
-        u->skip1 = unet_alloc<uint16_t>(u, 16 * n);
-        u->m1 = unet_alloc<uint16_t>(u, 32 * nm);
-        u->m2 = unet_alloc<uint16_t>(u, 32 * nm);
-        u->skip2 = unet_alloc<uint16_t>(u, 32 * nm);
-        u->c1 = unet_alloc<uint16_t>(u, 64 * nc);
-        u->c2 = unet_alloc<uint16_t>(u, 64 * nc);
-        u->resid = unet_alloc<float>(u, n);
-        u->dir = unet_alloc<float>(u, n);
-        u->guess = unet_alloc<float>(u, n);
-        u->norm_partials = unet_alloc<double>(u, (size_t)kPcgBlocks);
-        u->step_partials = unet_alloc<double>(u, 2 * (size_t)kPcgBlocks);
-    }
+    unet_prepare(c, u);
     DS_HIP(hipStreamSynchronize(c->stream));   // the host staging vectors go out of scope
 }
 
-bool unet_loaded(const ds_context* c) { return c->unet != nullptr && c->unet->in4 != nullptr; }
+bool unet_loaded(const ds_context* c) { return c->unet != nullptr && c->unet->head_w != nullptr; }
 
 // The network's guess for A x = b (device b, 0 off the unknowns), corrected from the base guess x_b
 // when given: returns the device guess buffer.
@@ -243,8 +379,10 @@ const float* unet_infer(ds_context* c, const float* b, const float* xb) {
     UNet* u = c->unet;
     require(unet_loaded(c), "no warm-start network loaded (ds_warm_start_load)");
     const Geo& g = c->geo;
-    const Dims d1{g.nx, g.ny, g.nz}, d2{g.nx / 2, g.ny / 2, g.nz / 2}, d3{g.nx / 4, g.ny / 4, g.nz / 4};
-    const uint64_t n1 = g.n, n2 = n1 / 8, n3 = n1 / 64;
+    unet_prepare(c, u);
+    const int L = u->levels, f = 1 << L;
+    const Dims d1 = u->dn, d2{d1.nx / 2, d1.ny / 2, d1.nz / 2}, d3{d1.nx / 4, d1.ny / 4, d1.nz / 4};
+    const uint64_t n1 = (uint64_t)d1.nx * d1.ny * d1.nz, n2 = n1 / 8, n3 = n1 / 64;
     const uint32_t per = pcg_periodic_bits(c);
     hipStream_t s = c->stream;
     const float* r = b;
@@ -252,9 +390,17 @@ const float* unet_infer(ds_context* c, const float* b, const float* xb) {
         base_residual_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g, per, c->code, b, xb, u->resid);
         r = u->resid;
     }
-    residual_norm_kernel<<<dim3(kPcgBlocks, 1), kBlock, 0, s>>>((uint32_t)n1, r, c->code, u->norm_partials);
-    pack_batch_input_kernel<<<dim3(kPcgBlocks, 1), kBlock, 0, s>>>((uint32_t)n1, c->unknowns, r, c->code, u->norm_partials, kPcgBlocks,
-                                                                   u->in4);
+    // the network's input: r itself, or r restricted to the network's grid
+    const float* rn = r;
+    const uint8_t* coden = c->code;
+    if (L > 0) {
+        restrict_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g, f, d1, r, u->rc);
+        rn = u->rc;
+        coden = u->ccode;
+    }
+    residual_norm_kernel<<<dim3(kPcgBlocks, 1), kBlock, 0, s>>>((uint32_t)n1, rn, coden, u->norm_partials);
+    pack_batch_input_kernel<<<dim3(kPcgBlocks, 1), kBlock, 0, s>>>((uint32_t)n1, u->net_unknowns, rn, coden, u->norm_partials,
+                                                                   kPcgBlocks, u->in4);
     constexpr int MT = 4;
     auto conv_waves = [&](const Dims& d) { return wave_blocks((uint64_t)(d.nx / 16) * (d.ny / MT) * d.nz); };
     auto fwd = [&](int k, uint16_t* out) {
@@ -279,9 +425,10 @@ const float* unet_infer(ds_context* c, const float* b, const float* xb) {
     tconv2_kernel<32, 16><<<wave_blocks((uint64_t)(d2.nx / 16) * d2.ny * d2.nz), 256, 0, s>>>(d2, 1, u->m1, u->wpk[9], u->bias[9], u->f1);
     conv3_kernel<16, 16, 16, MT, kEpiBiasRelu><<<conv_waves(d1), 256, 0, s>>>(d1, 1, u->f1, u->skip1, u->wpk[10], fwd(10, u->f2));
     conv3_kernel<16, 0, 16, MT, kEpiBiasRelu><<<conv_waves(d1), 256, 0, s>>>(d1, 1, u->f2, nullptr, u->wpk[11], fwd(11, u->f1));
-    head_kernel<<<blocks_for(n1), kBlock, 0, s>>>((uint32_t)n1, u->f1, u->head_w, u->bias[12], c->code, u->dir);
+    head_kernel<<<blocks_for(n1), kBlock, 0, s>>>((uint32_t)n1, u->f1, u->head_w, u->bias[12], coden, L > 0 ? u->dc : u->dir);
+    if (L > 0) prolong_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g, f, d1, c->code, u->dc, u->dir);
     step_partials_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g, per, c->code, r, u->dir, u->step_partials);
-    step_apply_kernel<<<kPcgBlocks, kBlock, 0, s>>>((uint32_t)n1, u->step_partials, kPcgBlocks, xb, u->dir, u->guess);
+    step_apply_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g.n, u->step_partials, kPcgBlocks, xb, u->dir, u->guess);
     check_launch("warm-start network");
     return u->guess;
 }

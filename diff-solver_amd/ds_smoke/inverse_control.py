@@ -34,8 +34,7 @@ def run(grid: int = 128, iterations: int = 200, steps: int = 64, k: int = 8, lr:
     if warm_start in ("cnn", "cnn_alone"):
         from . import warm_start as ws
 
-        params_flat, output_scale, _ = ws.load(weights)
-        ctx.warm_start_load(params_flat, output_scale)
+        ws.load_into(ctx, weights)
     s0 = ctx.state_from_host(scenes.initial_state(sc))
     control = ctx.control_from_host(scenes.control(sc))
     params = ctx.allocate_parameters(sc.params)

@@ -124,6 +124,7 @@ def load_library(path: str | None = None):
         "ds_warm_start_load": (C.c_int32, [P, P, C.c_uint64, C.c_double]),
         "ds_warm_start_infer": (C.c_int32, [P, P, P]),
         "ds_warm_start_guess": (C.c_int32, [P, P, P, P]),
+        "ds_warm_start_configure": (C.c_int32, [P, C.c_uint32]),
         "ds_warm_start_pair": (C.c_int32, [P, P, P, P]),
         "ds_unet_trainer_create": (C.c_int32, [P, C.c_uint32, P, C.c_uint64, C.POINTER(P)]),
         "ds_unet_trainer_destroy": (C.c_int32, [P]),
@@ -546,8 +547,10 @@ class Context:
             self._check(st, "ds_pcg_solve")
         return x, info, st
 
-    def warm_start_load(self, params: np.ndarray, output_scale: float):
-        """Load the warm-start network's FP32 parameters (state_dict order) and its output scale."""
+    def warm_start_load(self, params: np.ndarray, output_scale: float = 1.0, levels: int = 0):
+        """Load the warm-start network's FP32 parameters (state_dict order); levels: run it on the
+        grid coarsened 2^levels times (ds_warm_start_configure)."""
+        self._check(self.lib.ds_warm_start_configure(self.h, int(levels)), "ds_warm_start_configure")
         flat = np.ascontiguousarray(params, dtype=np.float32).ravel()
         self._check(self.lib.ds_warm_start_load(self.h, flat.ctypes.data_as(C.c_void_p), flat.size, float(output_scale)),
                     "ds_warm_start_load")

@@ -160,7 +160,7 @@ def train(ctx, R, params, batch=8, lr=1e-3, minutes=8.0, epochs=1000, seed=0, lo
     return out, history, step
 
 
-def evaluate(weights, grid, runs, steps, seed, device=0, log=print, solids=0):
+def evaluate(weights, grid, runs, steps, seed, device=0, log=print, solids=0, levels=0):
     """Held-out plumes at `grid`: for each forward step's problem A x = b (b = A p, p the step's
     pressure to 1e-8), CG iterations to 1e-6 from x0 = 0, the network's guess, the extrapolation of
     the previous solutions and the network's correction of it."""
@@ -175,7 +175,7 @@ def evaluate(weights, grid, runs, steps, seed, device=0, log=print, solids=0):
         sc = jittered_plume(grid, run, seed, 8.0 * grid / 64.0, solids)
         ctx = Context(sc.config(tolerance=1e-8, warm_start=abi.WARM_START_EXTRAPOLATE), abi.MODE_FORWARD, device)
         net = Context(sc.config(tolerance=1e-6, warm_start=abi.WARM_START_CNN_ALONE), abi.MODE_FORWARD, device)
-        net.warm_start_load(weights, 1.0)
+        net.warm_start_load(weights, 1.0, levels=levels)
         plain = Context(sc.config(tolerance=1e-6), abi.MODE_FORWARD, device)
         params = ctx.allocate_parameters(sc.params)
         ctl = ctx.control_from_host(scenes.control(sc))
@@ -258,8 +258,10 @@ def main(argv=None):
     ctx.close()
     del R
     if args.eval_runs > 0:
-        ev = evaluate(flat, args.eval_grid, args.eval_runs, args.eval_steps, seed=900, log=log, solids=args.solids)
-        print(json.dumps({"phase": "eval", "grid": args.eval_grid, **ev}), flush=True)
+        ev = evaluate(flat, args.eval_grid, args.eval_runs, args.eval_steps, seed=900, log=log, solids=args.solids,
+                      levels=warm_start.levels_for({"grid": args.grid}, (args.eval_grid,) * 3))
+        print(json.dumps({"phase": "eval", "grid": args.eval_grid, "levels": warm_start.levels_for({"grid": args.grid}, (args.eval_grid,) * 3),
+                          **ev}), flush=True)
 
 
 if __name__ == "__main__":

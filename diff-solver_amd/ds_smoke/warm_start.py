@@ -86,6 +86,21 @@ def load(path: str):
     return d["params"], float(d["output_scale"]), json.loads(str(d["meta"]))
 
 
+def levels_for(meta: dict, resolution) -> int:
+    """Coarsenings (ds_warm_start_configure) that bring a grid to the network's training grid:
+    log2(nx / training nx) when that is 0, 1 or 2, else 0."""
+    trained = int(meta.get("grid", resolution[0]))
+    ratio = int(resolution[0]) // max(1, trained)
+    return {1: 0, 2: 1, 4: 2}.get(ratio, 0) if ratio * trained == int(resolution[0]) else 0
+
+
+def load_into(ctx, path: str) -> dict:
+    """Load the weights file into a context, coarsening its grid to the training grid."""
+    flat, scale, meta = load(path)
+    ctx.warm_start_load(flat, scale, levels=levels_for(meta, ctx.n))
+    return meta
+
+
 def apply_matrix(x, diag, per=(False, False, False)):
     """A x for the pressure matrix on a (..., nz, ny, nx) tensor that is 0 off the unknowns
     (diag = the per-cell diagonal count, 0 off the unknowns)."""

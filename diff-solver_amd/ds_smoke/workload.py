@@ -32,16 +32,15 @@ def run(grid=256, solids=8, steps=32, k=8, modes=("none", "previous", "cnn"), we
 
     sc = scenes.plume(grid, solids=solids)
     warm = {"none": abi.WARM_START_NONE, "previous": abi.WARM_START_PREVIOUS, "extrapolate": abi.WARM_START_EXTRAPOLATE, "projection": abi.WARM_START_PROJECTION,
-            "cnn": abi.WARM_START_CNN}
+            "cnn_alone": abi.WARM_START_CNN_ALONE, "cnn": abi.WARM_START_CNN}
     s0h, kh = scenes.initial_state(sc), scenes.control(sc)
     results, grads = {}, {}
     for mode in modes:
         ctx = Context(sc.config(tolerance=tolerance, warm_start=warm[mode]), abi.MODE_DIFFERENTIABLE, device)
-        if mode == "cnn":
+        if mode in ("cnn", "cnn_alone"):
             from . import warm_start as ws
 
-            flat, scale, _ = ws.load(weights)
-            ctx.warm_start_load(flat, scale)
+            ws.load_into(ctx, weights)
         s0, ctl = ctx.state_from_host(s0h), ctx.control_from_host(kh)
         params = ctx.allocate_parameters(sc.params)
         target = ctx.to_device(sc.target)

@@ -100,12 +100,13 @@ enum { DS_CG_SINGLE_REDUCTION = 0, DS_CG_TWO_REDUCTION = 1, DS_CG_SINGLE_REDUCTI
  * restarts the sequence, and so does every trajectory entry point. */
 /* DS_WARM_START_PROJECTION: each forward and each adjoint solve starts from the A-norm-best
  * combination of the last 8 solutions of its sequence (Galerkin projection onto their span). */
-/* DS_WARM_START_CNN: the warm-start network corrects the sequence guess of DS_WARM_START_EXTRAPOLATE
- * (forward solves: the extrapolation; adjoint solves: the previous adjoint solution): with x_b that
- * guess, the network maps the residual r = b - A x_b to a direction d and the solve starts from
- * x_b + gamma d, gamma = <d, r> / <d, A d> (the A-norm-best step along d), never worse in the
- * A-norm than x_b alone. A solve without a base guess (the first of a sequence, the per-operator
- * projection entry points) starts from the network's direction for b alone, at its best multiple.
+/* DS_WARM_START_CNN: the warm-start network corrects the forward solves' sequence guess of
+ * DS_WARM_START_EXTRAPOLATE: with x_b that guess, the network maps the residual r = b - A x_b to a
+ * direction d and the solve starts from x_b + gamma d, gamma = <d, r> / <d, A d> (the A-norm-best
+ * step along d), never worse in the A-norm than x_b alone. Adjoint solves start from the previous
+ * adjoint solution, uncorrected. A solve without a base guess (the first of a sequence, the
+ * per-operator projection entry points) starts from the network's direction for b alone, at its
+ * best multiple.
  * DS_WARM_START_CNN_ALONE: every solve starts from the network's direction for b alone. */
 enum {
     DS_WARM_START_NONE = 0,
@@ -370,6 +371,12 @@ uint64_t ds_warm_start_parameter_count(void);
  * at its A-norm-best multiple, so a scale of the network's output cancels. A non-finite guess is
  * rejected (x0 = 0). */
 ds_status ds_warm_start_load(ds_context* c, const float* params, uint64_t count, double output_scale);
+/* The grid the network runs on: the solve's grid coarsened 2^levels times per axis (levels 0, 1
+ * or 2; default 0). With levels > 0 the network's input is restricted (sum over each coarse cell's
+ * children) and its output prolonged back (cell-centred trilinear), so a network trained on 64^3
+ * problems serves 128^3 (levels 1) or 256^3 (levels 2). The network's grid needs nx % 64 == 0,
+ * ny % 16 == 0, nz % 4 == 0. Callable before or after ds_warm_start_load. */
+ds_status ds_warm_start_configure(ds_context* c, uint32_t levels);
 /* The network's guess x for A x = b from b alone (device cell fields; b is 0 off the unknowns). */
 ds_status ds_warm_start_infer(ds_context* c, const float* b, float* x);
 /* The network's correction of a base guess: x = base + gamma N(b - A base) at the A-norm-best

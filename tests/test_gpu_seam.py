@@ -93,16 +93,28 @@ def test_step_across_tiles(seam):
     gc = ctx.to_host(cache)
     p = gc["pressure"][0].astype(np.float64) + gc["pressure"][1]
     ep = rel(p, ref_cache["pressure"])
-    # the projection alone, from the GPU's own advected velocity: the seam's share of the step
     adv = [rel(gc["advected_velocity"][a], ref_cache["advected_velocity"][a]) for a in range(3)]
-    _, p_own, _, _ = orc.projection_forward(tuple(v.astype(np.float64) for v in gc["advected_velocity"]))
-    e_own = rel(p, p_own)
-    print(f"  step periodic_x={periodic}: state {max(errs):.2e}, advected velocity {max(adv):.2e}, "
-          f"pressure {ep:.2e} (from the GPU's advected velocity {e_own:.2e}); solve: {info.iterations} iterations, "
-          f"final residual {info.final_residual / max(info.rhs_norm, 1e-300):.2e} relative")
+    # The projection alone, from the GPU's own advected velocity, through the projection operator
+    # (same kernels as the step): its FP32 right-hand side against the oracle's, and the residual
+    # of its solution under the oracle's matrix (a seam mistake would solve another system). The
+    # solution's error itself is not bounded by the residual along the near-null mode of the
+    # single-anchor matrix (eigenvalue ~1e-4 here): a 1e-8 residual left 3.3e-5 there.
+    ua = tuple(ctx.to_device(v) for v in gc["advected_velocity"])
+    _, p_op, rhs_op, info_op = ctx.projection_forward(ua)
+    _, code = ctx.masks()
+    rhs_gpu = np.where(code > 0, ctx.to_host(rhs_op).astype(np.float64), 0.0)
+    rhs_ref = np.where(code > 0, orc.pressure_rhs(tuple(v.astype(np.float64) for v in gc["advected_velocity"])), 0.0)
+    e_rhs = rel(rhs_gpu, rhs_ref)
+    res_op = np.linalg.norm(rhs_gpu - orc.apply_matrix(ctx.to_host(p_op).astype(np.float64))) / np.linalg.norm(rhs_gpu)
+    print(f"  step periodic_x={periodic}: state {max(errs):.2e}, advected velocity {max(adv):.2e}, pressure {ep:.2e}; "
+          f"projection operator: FP32 rhs {e_rhs:.2e}, residual under the oracle's matrix {res_op:.2e} "
+          f"({info_op.iterations} iterations to 1e-8)")
     assert max(errs) < 1e-5, errs
     assert max(adv) < 5e-5, adv
-    assert e_own < 3e-7, e_own
-    # FP32 against FP64 advection beside the solids moves the right-hand side, which the solve
-    # spreads: the same bound as the single-tile step tests
+    assert e_rhs < 1e-6, e_rhs
+    assert res_op < 2e-7, res_op   # 1e-8 plus the FP32 rounding of the returned solution
+    # The FP32 rounding of the right-hand side (4.5e-8 relative) reaches the pressure through the
+    # same near-null mode: 2.1e-5 (closed) and 1.7e-5 (periodic) when the FP64 oracle solves the
+    # FP32-rounded right-hand side instead of its own; measured here 2.4e-5 / 2.9e-5. The bound of
+    # the single-tile step tests:
     assert ep < 1e-4, ep

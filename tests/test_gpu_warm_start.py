@@ -77,3 +77,62 @@ def test_cnn_warm_start_needs_a_loaded_network():
         fresh.pcg_solve(fresh.to_device(b))
     with pytest.raises(SolverError):
         fresh.warm_start_load(flat[:-1], 1.0)
+
+
+def _prolong(dc, f, shape):
+    """Cell-centred trilinear interpolation of a coarse field to the fine grid (numpy), clamped."""
+    out = dc.astype(np.float64)
+    for axis, n in zip((0, 1, 2), shape):   # (z, y, x)
+        nc = out.shape[axis]
+        X = (np.arange(n) + 0.5) / f - 0.5
+        fl = np.floor(X)
+        t = X - fl
+        i0 = np.clip(fl.astype(int), 0, nc - 1)
+        i1 = np.clip(fl.astype(int) + 1, 0, nc - 1)
+        a, b = np.take(out, i0, axis=axis), np.take(out, i1, axis=axis)
+        tt = t.reshape([-1 if k == axis else 1 for k in range(3)])
+        out = a + tt * (b - a)
+    return out
+
+
+def test_network_on_the_coarsened_grid():
+    """levels = 1 at 128^3: restrict (sum of 8 children), the network at 64^3, trilinear prolongation,
+    the A-norm-best step, against the same chain assembled from the 64^3 network's own guess and
+    numpy; and the guess never increases the A-norm error of the base guess."""
+    import oracle as O
+    from ds_smoke import abi, scenes
+    from ds_smoke.solver import Context
+    from ds_smoke.unet import init_parameters
+
+    params = init_parameters(11)
+    sc = scenes.plume(128, with_target=False)
+    ctx = Context(sc.config(tolerance=1e-6, warm_start=abi.WARM_START_CNN_ALONE), abi.MODE_FORWARD)
+    ctx.warm_start_load(params, 1.0, levels=1)
+    sc64 = scenes.plume(64, with_target=False)
+    c64 = Context(sc64.config(tolerance=1e-6, warm_start=abi.WARM_START_CNN_ALONE), abi.MODE_FORWARD)
+    c64.warm_start_load(params, 1.0)
+    _, code = ctx.masks()
+    unk = code > 0
+    rng = np.random.default_rng(128)
+    # a smooth right-hand side (the solver's are): a few Gaussian bumps
+    z, y, x = np.meshgrid(*(np.arange(128) + 0.5,) * 3, indexing="ij")
+    b = sum(rng.uniform(-1, 1) * np.exp(-((x - cx) ** 2 + (y - cy) ** 2 + (z - cz) ** 2) / (2 * s * s))
+            for cx, cy, cz, s in rng.uniform(20, 108, (5, 4)) * [1, 1, 1, 0.15])
+    b = np.where(unk, b, 0.0).astype(np.float32)
+    got = ctx.to_host(ctx.warm_start_guess(ctx.to_device(b))).astype(np.float64)
+    rc = b.astype(np.float64).reshape(64, 2, 64, 2, 64, 2).sum((1, 3, 5)).astype(np.float32)
+    d64 = c64.to_host(c64.warm_start_infer(c64.to_device(rc))).astype(np.float64)
+    d = np.where(unk, _prolong(d64, 2, b.shape), 0.0)
+    orc = O.Oracle(sc, tolerance=1e-6)
+    ad = orc.apply_matrix(d)
+    ref = d * (np.dot(d.ravel(), b.astype(np.float64).ravel()) / np.dot(d.ravel(), ad.ravel()))
+    err = rel(got, ref)
+    print(f"levels 1 at 128^3 against restrict / 64^3 network / prolong: relative L2 {err:.2e}")
+    assert np.all(got[~unk] == 0)
+    assert err < 2e-2, err
+    # a base guess: the network's correction never raises the A-norm error (<e, A e> = x'Ax - 2 x'b + const)
+    base = (0.5 * ref).astype(np.float32)
+    corr = ctx.to_host(ctx.warm_start_guess(ctx.to_device(b), base=ctx.to_device(base))).astype(np.float64)
+    energy = lambda v: float(v.ravel() @ orc.apply_matrix(v).ravel() - 2.0 * v.ravel() @ b.astype(np.float64).ravel())
+    print(f"  energy: base {energy(base.astype(np.float64)):.6e}, corrected {energy(corr):.6e}")
+    assert energy(corr) <= energy(base.astype(np.float64)) + 1e-9 * abs(energy(base.astype(np.float64)))
