@@ -12,7 +12,9 @@ Stored (the full gradient is 24 MiB; these pin it at a few hundred KB):
   face DOFs (numpy default_rng(20261202).choice, sorted);
 * <g, v_j> for two directions v_j = default_rng(seed_j).standard_normal per face array (FP32),
   seeds 11 and 12, and the L2 norm of each axis' gradient;
-* one x-y plane (z = 64) of each axis' gradient (FP32).
+* two x-y planes of each axis' gradient (FP32): z = 40, off the scene's mirror planes, and z = 64,
+  the mirror plane x = z = 64 of the source crosses, where face velocities across it are near 0 and
+  traces end on the sampling planes (where the trilinear derivative jumps).
 
 Usage: python tests/golden/make_grad128.py [--threads T]   (about 20 minutes on 8 cores)
 """
@@ -32,7 +34,8 @@ from ds_smoke import scenes  # noqa: E402
 
 DOF_SEED = 20261202
 DIRECTION_SEEDS = (11, 12)
-PLANE_Z = 64
+PLANE_Z = 40
+MIRROR_Z = 64
 TOLERANCE = 1e-9
 
 
@@ -74,6 +77,9 @@ def main(threads):
         axis_norms=np.array([np.linalg.norm(v) for v in g]),
         plane_z=np.int64(PLANE_Z),
         plane_x=g[0][PLANE_Z].astype(np.float32), plane_y=g[1][PLANE_Z].astype(np.float32), plane_z_faces=g[2][PLANE_Z].astype(np.float32),
+        mirror_z=np.int64(MIRROR_Z),
+        mirror_x=g[0][MIRROR_Z].astype(np.float32), mirror_y=g[1][MIRROR_Z].astype(np.float32),
+        mirror_z_faces=g[2][MIRROR_Z].astype(np.float32),
         tolerance=np.float64(TOLERANCE),
         cg_iterations=np.array([info.forward_iterations, info.recompute_iterations, info.adjoint_iterations], dtype=np.int64),
         seconds=np.float64(dt), threads=np.int64(threads))
