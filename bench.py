@@ -40,6 +40,9 @@ BYTES_OTHER = {"pcg_direction_kernel": 1 + 4, "pcg_update_kernel": 1, "pcg_fused
 # algorithmic bytes of an iteration are the same (its timing unit is one iteration)
 BYTES_UNKNOWN["pcg_resident_kernel"] = BYTES_UNKNOWN["pcg_fused_kernel"]
 BYTES_OTHER["pcg_resident_kernel"] = BYTES_OTHER["pcg_fused_kernel"]
+# pcg_s2_kernel (s-step form, two iterations per launch): every cell reads the code, the five
+# basis vectors and the solution increment and writes the new basis and the increment
+BYTES_UNKNOWN["pcg_s2_kernel"] = BYTES_OTHER["pcg_s2_kernel"] = 1 + 5 * 4 + 4 + 5 * 4 + 4
 
 
 def host_threads():
@@ -160,10 +163,10 @@ def main():
                          "network modes when its weights exist)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "diff-solver_amd", "data", "warm_start_plume.npz"),
                     help="warm-start network parameters (for --warm-start cnn)")
-    ap.add_argument("--cg-variant", choices=["single", "two", "resident"], default="single",
+    ap.add_argument("--cg-variant", choices=["single", "two", "resident", "s2"], default="s2",
                     help="PCG recurrence: single-reduction (one fused kernel per iteration), resident (the same "
-                         "recurrence as one cooperative launch per replacement interval where the grid allows) "
-                         "or two-reduction")
+                         "recurrence as one cooperative launch per replacement interval where the grid allows), "
+                         "two-reduction, or s2 (s-step: two iterations per fused launch)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 10 device-to-device copies of MIB MiB (the reference for HBM counter scaling)")
     args = ap.parse_args()
@@ -180,7 +183,7 @@ def main():
 
     sc = scenes.plume(args.grid)
     variant = {"single": abi.CG_SINGLE_REDUCTION, "two": abi.CG_TWO_REDUCTION,
-               "resident": abi.CG_SINGLE_REDUCTION_RESIDENT}[args.cg_variant]
+               "resident": abi.CG_SINGLE_REDUCTION_RESIDENT, "s2": abi.CG_S_STEP}[args.cg_variant]
     if args.warm_start == "all":
         modes = ["none", "extrapolate"] + (["cnn_alone", "cnn"] if os.path.exists(args.weights) else [])
     else:

@@ -137,7 +137,8 @@ enum class WarmStart : std::uint32_t {
 enum class CgVariant : std::uint32_t {
     single_reduction = DS_CG_SINGLE_REDUCTION,
     two_reduction = DS_CG_TWO_REDUCTION,
-    single_reduction_resident = DS_CG_SINGLE_REDUCTION_RESIDENT
+    single_reduction_resident = DS_CG_SINGLE_REDUCTION_RESIDENT,
+    s_step = DS_CG_S_STEP
 };
 
 struct Configuration {
@@ -155,7 +156,7 @@ struct Configuration {
     double cg_tolerance{1e-6};
     std::uint32_t cg_max_iterations{0u};
     WarmStart warm_start{WarmStart::none};
-    CgVariant cg_variant{CgVariant::single_reduction};
+    CgVariant cg_variant{CgVariant::s_step};
 };
 
 struct ScalarField {

@@ -103,7 +103,7 @@ void validate(const ds_config& cfg, uint32_t mode) {
     if ((cfg.warm_start == DS_WARM_START_CNN || cfg.warm_start == DS_WARM_START_CNN_ALONE) && (cfg.resolution[0] % 64 || cfg.resolution[1] % 16 || cfg.resolution[2] % 4))
         throw Failure{DS_ERROR_UNSUPPORTED, "the warm-start network needs nx % 64 == 0, ny % 16 == 0 and nz % 4 == 0"};
     if (!(cfg.cg_tolerance >= 0.0) || !std::isfinite(cfg.cg_tolerance)) bad("cg_tolerance must be >= 0 and finite");
-    if (cfg.cg_variant > DS_CG_SINGLE_REDUCTION_RESIDENT) bad("cg_variant out of range");
+    if (cfg.cg_variant > DS_CG_S_STEP) bad("cg_variant out of range");
 }
 
 // model.cpp:56-182 plus the pressure connectivity (operators.cu:852-873): solid mask from cell
@@ -415,7 +415,18 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
         c->pcg.p[1] = device_alloc<float>(c, n + 1);
         for (float* v : {c->pcg.r, c->pcg.p[0], c->pcg.p[1]}) fill_zero(c, v, n + 1);
         // the resident single-reduction variant runs the same recurrence
-        c->pcg.variant = cfg.cg_variant == DS_CG_TWO_REDUCTION ? DS_CG_TWO_REDUCTION : DS_CG_SINGLE_REDUCTION;
+        c->pcg.variant = cfg.cg_variant == DS_CG_TWO_REDUCTION || cfg.cg_variant == DS_CG_S_STEP ? cfg.cg_variant : DS_CG_SINGLE_REDUCTION;
+        if (c->pcg.variant == DS_CG_S_STEP) {
+            // every basis vector carries the zero pad element n (read by out-of-domain rim loads)
+            for (int k = 0; k < 2; ++k)
+                for (int m = 0; m < 5; ++m) {
+                    c->pcg.s2v[k][m] = device_alloc<float>(c, n + 1);
+                    fill_zero(c, c->pcg.s2v[k][m], n + 1);
+                }
+            c->pcg.xd = device_alloc<float>(c, n + 1);
+            fill_zero(c, c->pcg.xd, n + 1);
+            c->pcg.s2_parts = device_alloc<double>(c, 2 * (size_t)16 * kPcgMaxBlocks);
+        }
         if (c->pcg.variant == DS_CG_SINGLE_REDUCTION) {
             // s is written on unknowns only; its other cells must read as 0 (as halo values)
             c->pcg.r_alt = device_alloc<float>(c, n + 1);

@@ -95,7 +95,7 @@ struct PcgStatus {
 
 // HIP-event timing of sampled PCG kernel launches (kernel 0 = direction, 1 = update, 2 = the
 // fused single-reduction iteration).
-constexpr int kProfiledKernels = 4;   // 3 = the resident single-reduction launch (units: iterations)
+constexpr int kProfiledKernels = 5;   // 3 = the resident single-reduction launch (units: iterations), 4 = the s-step launch
 struct Profiler {
     uint32_t every = 0;
     uint64_t launches[kProfiledKernels]{};
@@ -153,6 +153,9 @@ struct PcgWork {
     float* wv[2]{};
     float* sv[2]{};
     float* xd = nullptr;         // solution increment since the last fold into the pair
+    // s-step form (DS_CG_S_STEP): the basis P0, P1, P2, Z0, Z1 double-buffered; its partial sums
+    float* s2v[2][5]{};
+    double* s2_parts = nullptr;  // two sets of 16 * kPcgMaxBlocks
     double* partials = nullptr;  // two sets of kPartCount * kPcgMaxBlocks
     double* hist = nullptr;      // per-iteration scalars: rz[k], pq[k] (2 * (max_it + 2))
     PcgStatus* status = nullptr; // device

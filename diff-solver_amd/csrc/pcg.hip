@@ -29,6 +29,7 @@
 #include "pcg_fused.hpp"
 #include "pcg_plane.hpp"
 #include "pcg_resident.hpp"
+#include "pcg_s2.hpp"
 #include "rhs.hpp"
 
 namespace ds {
@@ -221,6 +222,93 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_fold_kernel(uint32_t n, float
         xl[i] = l;
         xd[i] = 0.0f;
     }
+}
+
+// s-step priming, pass 1 (one cell per thread, neighbours gathered): from the residual r and the
+// direction P0 of set v (WithP; else p = z), P1 = T p, Z0 = z = D^-1 r and Z1 = T z, written into
+// the same set (P0 too when p = z). T is evaluated exactly as in pcg_s2_kernel.
+template <bool WithP>
+__global__ __launch_bounds__(kBlock) void pcg_s2_prime_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
+                                                              const float* __restrict__ r, S2Vecs v, PcgStatus* status,
+                                                              uint32_t respect_stop) {
+    if (respect_stop && status->stop) return;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    const uint32_t d = code[i];
+    float* P0 = v.o[0];
+    if (d == 0u) {
+        if (!WithP) P0[i] = 0.0f;
+        v.o[1][i] = 0.0f;
+        v.o[3][i] = 0.0f;
+        v.o[4][i] = 0.0f;
+        return;
+    }
+    int x, y, z;
+    decode_cell(g, i, x, y, z);
+    const Stencil st = stencil(g, per, x, y, z, i);
+    auto zval = [&](uint32_t j) { return r[j] * inv_diag(code[j]); };
+    float zn[6], pn[6];
+#pragma unroll
+    for (int q = 0; q < 6; ++q) {
+        zn[q] = st.w[q] * zval(st.j[q]);
+        pn[q] = WithP ? st.w[q] * P0[st.j[q]] : zn[q];
+    }
+    const float df = (float)d, inv = inv_diag(d);
+    const float zc = zval(i);
+    const float pc = WithP ? P0[i] : zc;
+    const float tz = s2_apply(zc, zn[0], zn[1], zn[2], zn[3], zn[4], zn[5], df, inv);
+    const float tp = WithP ? s2_apply(pc, pn[0], pn[1], pn[2], pn[3], pn[4], pn[5], df, inv) : tz;
+    if (!WithP) P0[i] = zc;
+    v.o[1][i] = tp;
+    v.o[3][i] = zc;
+    v.o[4][i] = tz;
+}
+
+// s-step priming, pass 2: P2 = T P1 and the partial sums of the basis' D-weighted Gram matrix and
+// of rho = ||r||^2 (the residual that pass 1 read), in the s-step kernel's slot order.
+__global__ __launch_bounds__(kPcgThreads) void pcg_s2_gram_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
+                                                                  const float* __restrict__ r, S2Vecs v, double* __restrict__ out_parts,
+                                                                  PcgStatus* status, uint32_t respect_stop) {
+    __shared__ double lds[16 * kMaxWaves];
+    if (respect_stop && status->stop) return;
+    uint32_t lb, start, end;
+    block_range(g.n, lb, start, end);
+    double acc[kS2Parts];
+#pragma unroll
+    for (int q = 0; q < kS2Parts; ++q) acc[q] = 0.0;
+    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+        const uint32_t d = code[i];
+        float P2 = 0.0f;
+        if (d != 0u) {
+            int x, y, z;
+            decode_cell(g, i, x, y, z);
+            const Stencil st = stencil(g, per, x, y, z, i);
+            float qn[6];
+#pragma unroll
+            for (int q = 0; q < 6; ++q) qn[q] = st.w[q] * v.o[1][st.j[q]];
+            P2 = s2_apply(v.o[1][i], qn[0], qn[1], qn[2], qn[3], qn[4], qn[5], (float)d, inv_diag(d));
+        }
+        v.o[2][i] = P2;
+        if (d == 0u) continue;
+        const float nv[5]{v.o[0][i], v.o[1][i], P2, v.o[3][i], v.o[4][i]};
+        const double dd = (double)d;
+        double q[5], wq[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            q[m] = (double)nv[m];
+            wq[m] = dd * q[m];
+        }
+        int sidx = 0;
+#pragma unroll
+        for (int a = 0; a < 5; ++a)
+#pragma unroll
+            for (int c = a; c < 5; ++c) {
+                acc[sidx] = __fma_rn(wq[a], q[c], acc[sidx]);
+                ++sidx;
+            }
+        acc[kS2Rho] = __fma_rn((double)r[i], (double)r[i], acc[kS2Rho]);
+    }
+    s2_block_partials(acc, lds, out_parts + 16u * lb);
 }
 
 // One block: reduce the ||b||^2 (optional) and ||r||^2 partials into the status.
@@ -497,6 +585,135 @@ ds_solve_info pcg_iterate_fused(ds_context* c, float* xh, float* xl, double tol,
     return info;
 }
 
+// s-step iteration (DS_CG_S_STEP) from the state left by launch_setup (r, BB partials in set 0 of
+// w.partials) until the true residual meets tol or max_it. Each launch performs two iterations (one
+// when a single iteration is left before max_it); every replacement interval (rounded up to even)
+// folds the increment, recomputes the true residual and primes a new basis with the same direction.
+ds_solve_info pcg_iterate_s2(ds_context* c, float* xh, float* xl, double tol, uint32_t max_it, bool warm, bool rejected) {
+    PcgWork& w = c->pcg;
+    const uint32_t per = periodic_bits(c);
+    const double tol2 = tol * tol;
+    const uint32_t R = (c->replacement_interval + 1u) & ~1u;
+    double* parts[2]{w.s2_parts, w.s2_parts + (size_t)16 * kPcgMaxBlocks};
+    auto vecs = [&](int set) {
+        S2Vecs v;
+        for (int m = 0; m < 5; ++m) {
+            v.v[m] = w.s2v[set][m];
+            v.o[m] = w.s2v[set ^ 1][m];
+        }
+        v.xd = w.xd;
+        return v;
+    };
+    auto prime = [&](int set, bool with_p, uint32_t respect_stop) {
+        S2Vecs v = vecs(set ^ 1);   // v.o = set
+        const uint32_t nb = xcd_blocks_for(c->geo.n);
+        if (with_p)
+            pcg_s2_prime_kernel<true><<<nb, kBlock, 0, c->stream>>>(c->geo, per, c->code, w.r, v, w.status, respect_stop);
+        else
+            pcg_s2_prime_kernel<false><<<nb, kBlock, 0, c->stream>>>(c->geo, per, c->code, w.r, v, w.status, respect_stop);
+        pcg_s2_gram_kernel<<<w.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, c->code, w.r, v, parts[set], w.status, respect_stop);
+    };
+    auto fold = [&](uint32_t respect_stop) {
+        pcg_fold_kernel<<<w.blocks, kPcgThreads, 0, c->stream>>>(c->geo.n, xh, xl, w.xd, w.status, respect_stop);
+    };
+    auto true_residual = [&](uint32_t respect_stop) {
+        pcg_true_residual_kernel<<<w.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, c->code, w.b, xh, xl, w.r, w.partials, w.status,
+                                                                          respect_stop);
+    };
+    auto launch = [&](int set, uint32_t k, uint32_t steps, uint32_t check_only) {
+        pcg_s2_kernel<<<w.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, w.tiling, c->code, vecs(set), parts[set], w.blocks,
+                                                               parts[set ^ 1], w.status, tol2, k, steps, check_only);
+    };
+    ds_solve_info info{};
+    info.warm_started = warm ? 1u : 0u;
+    info.guess_rejected = rejected ? 1u : 0u;
+    fill_zero(c, w.xd, c->geo.n);
+    // ||b||^2 (and ||r0||^2) from the set-up's partials into the status
+    pcg_finalize_kernel<<<1, kBlock, 0, c->stream>>>(w.partials, w.blocks, w.status, 1u);
+    int set = 0;
+    prime(0, false, 0u);
+    check_launch("pcg_s2_prime_kernel");
+    uint32_t k = 0;
+    uint32_t restarts = 0;
+    Profiler& prof = c->profiler;
+    // (iteration at launch, set read) of this pass's launches, to find the state a stop left
+    std::vector<std::pair<uint32_t, int>> launched;
+    while (true) {
+        uint32_t batch = 1;
+        int slot = 0, pending = -1;
+        bool stop_seen = false;
+        launched.clear();
+        while (k < max_it && !stop_seen) {
+            for (uint32_t j = 0; j < batch && k < max_it; ++j) {
+                const uint32_t steps = std::min<uint32_t>(2u, max_it - k);
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (prof.every && (k / 2u) % prof.every == 0 && prof.used + 2 <= 6000) {
+                    e0 = prof.take();
+                    e1 = prof.take();
+                }
+                if (e0) DS_HIP(hipEventRecord(e0, c->stream));
+                launch(set, k, steps, 0u);
+                if (e1) {
+                    DS_HIP(hipEventRecord(e1, c->stream));
+                    prof.recs.push_back({4, e0, e1});
+                }
+                if (prof.every) prof.launches[4]++;
+                launched.push_back({k, set});
+                k += steps;
+                set ^= 1;
+                if (k % R == 0u && k < max_it) {
+                    fold(1u);
+                    true_residual(1u);
+                    prime(set, true, 1u);
+                }
+            }
+            check_launch("pcg_s2_kernel");
+            DS_HIP(hipMemcpyAsync(&w.host_status[slot], w.status, sizeof(PcgStatus), hipMemcpyDeviceToHost, c->stream));
+            DS_HIP(hipEventRecord(w.batch_event[slot], c->stream));
+            if (pending >= 0) {
+                DS_HIP(hipEventSynchronize(w.batch_event[pending]));
+                if (w.host_status[pending].stop) stop_seen = true;
+            }
+            pending = slot;
+            slot ^= 1;
+            batch = std::min<uint32_t>(batch * 2u, 16u);
+        }
+        PcgStatus st = read_status(c);
+        if (!st.stop && k >= max_it) {
+            // one more convergence test on the last launch's residual
+            launch(set, k, 2u, 1u);
+            check_launch("pcg_s2_kernel (check)");
+            st = read_status(c);
+            launched.push_back({k, set});
+        }
+        // the state a stop left is the set its launch read (the launches queued after it returned
+        // at once)
+        const uint32_t done = st.stop ? st.iterations : k;
+        for (const auto& l : launched)
+            if (l.first == done) set = l.second;
+        k = done;
+        fold(0u);
+        true_residual(0u);
+        pcg_finalize_kernel<<<1, kBlock, 0, c->stream>>>(w.partials, w.blocks, w.status, 0u);
+        check_launch("pcg_true_residual_kernel");
+        const PcgStatus fin = read_status(c);
+        info.iterations = done;
+        info.rhs_norm = std::sqrt(fin.bb);
+        info.final_residual = std::sqrt(std::max(0.0, fin.rr));
+        const bool finite = std::isfinite(fin.rr);
+        const bool ok = finite && fin.rr <= tol2 * fin.bb;
+        info.converged = ok ? 1u : 0u;
+        if (!finite) info.final_residual = fin.rr;
+        if (ok || !finite || !st.stop || done >= max_it || ++restarts > 8) break;
+        // the recursive residual met the tolerance but the true one does not: continue from the
+        // true residual (computed above) with the same direction
+        prime(set, done > 0, 0u);
+        pcg_clear_stop_kernel<<<1, 64, 0, c->stream>>>(w.status);
+        check_launch("pcg_clear_stop_kernel");
+    }
+    return info;
+}
+
 // Iterates from the state left by launch_setup until the true residual meets tol or max_it.
 ds_solve_info pcg_iterate(ds_context* c, float* xh, float* xl, double tol, uint32_t max_it, bool warm, bool rejected) {
     PcgWork& w = c->pcg;
@@ -664,7 +881,8 @@ ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], c
         }
     }
     ds_solve_info info = c->pcg.variant == DS_CG_SINGLE_REDUCTION ? pcg_iterate_fused(c, xh, xl, tol, max_it, warm, rejected)
-                                                                  : pcg_iterate(c, xh, xl, tol, max_it, warm, rejected);
+                         : c->pcg.variant == DS_CG_S_STEP        ? pcg_iterate_s2(c, xh, xl, tol, max_it, warm, rejected)
+                                                                 : pcg_iterate(c, xh, xl, tol, max_it, warm, rejected);
     info.initial_residual = warm && !rejected ? std::sqrt(std::max(0.0, rr0)) : info.rhs_norm;
     (void)bb;
     if (win && info.converged) window_add(c, *win, xh, xl);
@@ -744,7 +962,8 @@ ds_status ds_profile_read(ds_context* c, ds_kernel_timing* out, uint32_t capacit
             samples[r.kernel] += r.units;
         }
         // the resident launch is reported per iteration: launches and samples count iterations
-        const char* names[kProfiledKernels]{"pcg_direction_kernel", "pcg_update_kernel", "pcg_fused_kernel", "pcg_resident_kernel"};
+        const char* names[kProfiledKernels]{"pcg_direction_kernel", "pcg_update_kernel", "pcg_fused_kernel", "pcg_resident_kernel",
+                                            "pcg_s2_kernel"};
         uint32_t n = 0;
         for (int k = 0; k < kProfiledKernels; ++k) {   // kernels that ran
             if (prof.launches[k] == 0 || n >= capacity) continue;

@@ -80,16 +80,23 @@ typedef struct ds_collider {
  * (operators.cu:885-902,1004-1013), kept as a reference mode. */
 enum { DS_PROJECTION_PCG = 0, DS_PROJECTION_RBGS = 1 };
 
-/* PCG recurrence: single-reduction (Chronopoulos-Gear: one fused kernel and one global reduction
- * per iteration; the default) or the two-reduction textbook form (two kernels per iteration).
- * Both iterate the same Jacobi-preconditioned CG; they differ only in rounding. */
-/* DS_CG_SINGLE_REDUCTION: one FP64 reduction and one fused kernel launch per iteration (default).
+/* PCG recurrence: s-step (two iterations per fused kernel and per global reduction; what the C++
+ * facade and the Python host select by default), single-reduction (Chronopoulos-Gear: one fused
+ * kernel and one global reduction per iteration; the value of a zero-initialised ds_config) or the
+ * two-reduction textbook form (two kernels per iteration). All iterate the same
+ * Jacobi-preconditioned CG; they differ only in rounding. */
+/* DS_CG_SINGLE_REDUCTION: one FP64 reduction and one fused kernel launch per iteration.
  * DS_CG_SINGLE_REDUCTION_RESIDENT: the same recurrence as one cooperative launch per
  * residual-replacement interval with the vectors held on chip, where the grid allows it (nx a
  * multiple of 64, ny of 8, at most 8 planes per block, every block co-resident), else as the
  * default; faster on small grids (64^3), slower at 128^3. DS_CG_TWO_REDUCTION: the classic
  * two-kernel form. */
-enum { DS_CG_SINGLE_REDUCTION = 0, DS_CG_TWO_REDUCTION = 1, DS_CG_SINGLE_REDUCTION_RESIDENT = 2 };
+/* DS_CG_S_STEP: the same Jacobi-preconditioned CG two iterations per kernel launch and per
+ * reduction (s-step CG, s = 2: the basis [p, T p, T^2 p, z, T z], T = D^-1 A, is stored and its
+ * D-weighted Gram matrix gives both iterations' scalars; about 1.7x the one-step form's iterations
+ * per second at 128^3). The stopping test runs every second iteration, so iteration counts are
+ * even except at an odd iteration cap. */
+enum { DS_CG_SINGLE_REDUCTION = 0, DS_CG_TWO_REDUCTION = 1, DS_CG_SINGLE_REDUCTION_RESIDENT = 2, DS_CG_S_STEP = 3 };
 
 /* Initial guess of the forward and adjoint pressure solves. */
 /* DS_WARM_START_EXTRAPOLATE: each forward pressure solve of a sweep starts from the quadratic

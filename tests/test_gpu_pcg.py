@@ -1,10 +1,12 @@
-"""Jacobi-PCG on the projection matrix (solvers/smoke/operators.cu:852-902), both recurrences:
-the single-reduction form (one fused kernel per iteration) and the two-reduction form.
+"""Jacobi-PCG on the projection matrix (solvers/smoke/operators.cu:852-902), three recurrences:
+the single-reduction form (one fused kernel per iteration), the two-reduction form and the s-step
+form (two iterations per fused kernel and per reduction).
 
 Checked against the FP64 oracle's solve of the same system on the three scenes, plus the
 solver's own contract: the true residual meets the tolerance, a guess equal to the solution
 stops at once, a non-finite guess is rejected, b = 0 gives x = 0, an iteration cap that is too
-small reports DS_ERROR_NOT_CONVERGED, and both recurrences need about the same iteration count.
+small reports DS_ERROR_NOT_CONVERGED (an odd cap included), and the recurrences need about the
+same iteration count.
 """
 import numpy as np
 import pytest
@@ -13,7 +15,9 @@ from helpers import SCENES, rel
 
 pytestmark = pytest.mark.gpu
 
-VARIANTS = {"single": 0, "two": 1}
+VARIANTS = {"single": 0, "two": 1, "s2": 3}
+# the recurrence each variant's iteration count is compared with
+OTHER = {"single": 1, "two": 0, "s2": 0}
 
 
 @pytest.fixture(scope="module", params=[(s, v) for s in sorted(SCENES) for v in sorted(VARIANTS)],
@@ -50,7 +54,7 @@ def test_variants_take_similar_iterations(solver):
     from ds_smoke import abi
     from ds_smoke.solver import Context
 
-    other = Context(sc.config(tolerance=1e-6, cg_variant=1 - VARIANTS[variant]), abi.MODE_FORWARD)
+    other = Context(sc.config(tolerance=1e-6, cg_variant=OTHER[variant]), abi.MODE_FORWARD)
     _, i1, _ = ctx.pcg_solve(ctx.to_device(b))
     _, i2, _ = other.pcg_solve(other.to_device(b))
     print(f"{scene}: {variant} {i1.iterations} iterations, other {i2.iterations}")

@@ -130,7 +130,9 @@ def test_loss_gradient_matches_oracle(k):
     errs = [rel(g["velocity"][a], ref["grad"]["velocity"][a]) for a in range(3)]
     e_par = rel(ctx.to_host(pa)[:3], ref["params"][:3])
     print(f"k = {k}: loss {abs(loss - ref['loss']) / ref['loss']:.2e}, velocity gradient {max(errs):.2e}, parameters {e_par:.2e}")
-    # measured (k = 1, 3, 8): velocity gradient 1.2e-5, parameter adjoints 8e-7 relative L2
+    # measured (k = 1, 3, 8): velocity gradient 1.2e-5, parameter adjoints 8e-7 relative L2; the
+    # smallest parameter adjoint (1% of the largest) within 8e-5 of itself (1e-6 solves)
     for a in range(3):
         assert errs[a] < 1e-4, (a, errs[a])
-    np.testing.assert_allclose(ctx.to_host(pa)[:3], ref["params"][:3], rtol=2e-5)
+    assert e_par < 5e-6, e_par
+    np.testing.assert_allclose(ctx.to_host(pa)[:3], ref["params"][:3], rtol=2e-4)
