@@ -417,12 +417,14 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
         // the resident single-reduction variant runs the same recurrence
         c->pcg.variant = cfg.cg_variant == DS_CG_TWO_REDUCTION || cfg.cg_variant == DS_CG_S_STEP ? cfg.cg_variant : DS_CG_SINGLE_REDUCTION;
         if (c->pcg.variant == DS_CG_S_STEP) {
-            // every basis vector carries the zero pad element n (read by out-of-domain rim loads)
-            for (int k = 0; k < 2; ++k)
-                for (int m = 0; m < 5; ++m) {
-                    c->pcg.s2v[k][m] = device_alloc<float>(c, n + 1);
-                    fill_zero(c, c->pcg.s2v[k][m], n + 1);
-                }
+            // every basis vector carries the zero pad element n (read by out-of-domain rim loads);
+            // a set is addressed with 32-bit byte offsets
+            if (5ull * (n + 1) * sizeof(float) >= (1ull << 32)) throw Failure{DS_ERROR_UNSUPPORTED, "grid too large for the s-step PCG"};
+            for (int k = 0; k < 2; ++k) {
+                c->pcg.s2set[k] = device_alloc<float>(c, 5 * (size_t)(n + 1));
+                fill_zero(c, c->pcg.s2set[k], 5 * (size_t)(n + 1));
+                for (int m = 0; m < 5; ++m) c->pcg.s2v[k][m] = c->pcg.s2set[k] + (size_t)m * (n + 1);
+            }
             c->pcg.xd = device_alloc<float>(c, n + 1);
             fill_zero(c, c->pcg.xd, n + 1);
             c->pcg.s2_parts = device_alloc<double>(c, 2 * (size_t)16 * kPcgMaxBlocks);
@@ -441,7 +443,8 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
             }
         }
         {
-            // about kPcgTargetBlocks blocks: zc = tiles * nz / kPcgTargetBlocks rounded up to a power of two in [4, 64]
+            // about kPcgTargetBlocks blocks (kPcgTargetBlocksS2 for the s-step form): zc = tiles * nz / target
+            // rounded up to a power of two in [4, 64]
             PlaneTiling& t = c->pcg.tiling;
             t.tiles_x = (uint32_t)((g.nx + kPlaneTileX - 1) / kPlaneTileX);
             t.tiles_y = (uint32_t)((g.ny + kPlaneTileY - 1) / kPlaneTileY);
@@ -449,7 +452,8 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
             uint32_t zc = 4;
             // the fused kernel reads one producer partial per thread: blocks <= kPcgThreads
             const uint64_t cap = std::min<uint64_t>(kPcgMaxBlocks, kPcgThreads);
-            while (zc < 64 && tiles * (uint64_t)g.nz > kPcgTargetBlocks * zc) zc *= 2;
+            const uint64_t target = cfg.cg_variant == DS_CG_S_STEP ? kPcgTargetBlocksS2 : kPcgTargetBlocks;
+            while (zc < 64 && tiles * (uint64_t)g.nz > target * zc) zc *= 2;
             while (tiles * (((uint64_t)g.nz + zc - 1) / zc) > cap) zc *= 2;
             t.zc = zc;
             const uint64_t G = tiles * (((uint64_t)g.nz + zc - 1) / zc);

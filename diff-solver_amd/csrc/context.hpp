@@ -73,6 +73,8 @@ constexpr uint32_t kMetricBlocks = 512;
 constexpr uint32_t kPcgThreads = 512;
 // Block count the plane-marching iteration kernels aim for (the z chunk is sized to it).
 constexpr uint64_t kPcgTargetBlocks = 512;
+// the s-step kernel: one 512-thread block per CU, each with a longer z chunk and deeper prefetch
+constexpr uint64_t kPcgTargetBlocksS2 = 256;
 constexpr uint32_t kPcgUnroll = 4;
 constexpr uint32_t kPcgMaxBlocks = 1024;
 
@@ -154,7 +156,8 @@ struct PcgWork {
     float* sv[2]{};
     float* xd = nullptr;         // solution increment since the last fold into the pair
     // s-step form (DS_CG_S_STEP): the basis P0, P1, P2, Z0, Z1 double-buffered; its partial sums
-    float* s2v[2][5]{};
+    float* s2set[2]{};           // each 5 (n + 1) floats: P0, P1, P2, Z0, Z1
+    float* s2v[2][5]{};          // the vectors of each set
     double* s2_parts = nullptr;  // two sets of 16 * kPcgMaxBlocks
     double* partials = nullptr;  // two sets of kPartCount * kPcgMaxBlocks
     double* hist = nullptr;      // per-iteration scalars: rz[k], pq[k] (2 * (max_it + 2))

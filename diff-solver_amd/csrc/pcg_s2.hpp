@@ -35,7 +35,7 @@ namespace ds {
 constexpr int kS2Parts = 16;
 constexpr int kS2Rho = 15;
 // Planes whose loads are in flight ahead of stage A.
-constexpr int kS2Depth = 1;
+constexpr int kS2Depth = 3;
 // Steps of the plane march per unrolled loop pass: one load slot each.
 constexpr int kS2Slots = kS2Depth + 1;
 
@@ -46,6 +46,26 @@ struct S2Vecs {
     float* o[5];         // basis for the next launch (set c^1)
     float* xd;           // solution increment since the last fold into the pair (own cells, in place)
 };
+
+// The same sets as one allocation each: vector m of a set starts at element m * (n + 1). The
+// iteration kernel addresses them through buffer resources (a 32-bit byte offset per cell shared
+// by the five vectors), so a set must stay below 4 GiB: 5 (n + 1) 4 bytes < 2^32.
+struct S2Sets {
+    const float* in;     // set c
+    float* out;          // set c^1
+    float* xd;
+};
+constexpr uint32_t kS2BufferFlags = 0x00020000u;   // dword 3 of a raw (stride 0) gfx9 buffer resource
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t s2_resource(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, (int)kS2BufferFlags);
+}
+__device__ __forceinline__ float s2_load(__amdgpu_buffer_rsrc_t r, uint32_t byte, uint32_t base) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)byte, (int)base, 0));
+}
+__device__ __forceinline__ void s2_store(float v, __amdgpu_buffer_rsrc_t r, uint32_t byte, uint32_t base) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(int, v), r, (int)byte, (int)base, 0);
+}
 
 // The two (or one) CG steps in basis coordinates. a, c, e: coordinates of the new direction,
 // residual and solution increment; gamma = z^T D z before; ok = false when a denominator is not
@@ -173,15 +193,25 @@ __device__ __forceinline__ void s2_block_partials(const double (&v)[16], double*
     }
 }
 
-// Totals of the producer's per-block partials (block-major, 16 per block, `blocks` <= 32 * 16
-// blocks), identical in every consumer block: thread t sums slot t & 15 over blocks t >> 4, +32,
-// ...; then over the 4 lanes of a wave with that slot, then over the waves in a fixed order.
-// Returns with tot[0..15] in LDS, after a barrier.
-__device__ __forceinline__ void s2_totals(const double* __restrict__ parts, uint32_t blocks, double* lds, double* tot) {
+// Totals of the producer's per-block partials (block-major, 16 per block, at most 512 blocks),
+// identical in every consumer block: thread t sums slot t & 15 over blocks t >> 4, +32, ...
+// (s2_load_partials, issued before anything else of the launch), then over the 4 lanes of a wave
+// with that slot, then over the waves in a fixed order (s2_totals, which returns with tot[0..15]
+// in LDS after a barrier).
+constexpr int kS2PartLoads = (512 * 16) / kPcgThreads;
+__device__ __forceinline__ void s2_load_partials(const double* __restrict__ parts, uint32_t blocks, double (&pl)[kS2PartLoads]) {
+    const uint32_t q = threadIdx.x & 15u, b0 = threadIdx.x >> 4;
+#pragma unroll
+    for (int j = 0; j < kS2PartLoads; ++j) {
+        const uint32_t b = b0 + (uint32_t)j * (kPcgThreads / 16);
+        pl[j] = b < blocks ? parts[b * 16u + q] : 0.0;
+    }
+}
+__device__ __forceinline__ void s2_totals(const double (&pl)[kS2PartLoads], double* lds, double* tot) {
     constexpr int kWaves = (int)(kPcgThreads / 64);
-    const uint32_t q = threadIdx.x & 15u;
     double s = 0.0;
-    for (uint32_t b = threadIdx.x >> 4; b < blocks; b += kPcgThreads / 16) s += parts[b * 16u + q];
+#pragma unroll
+    for (int j = 0; j < kS2PartLoads; ++j) s += pl[j];
     s += __shfl_xor(s, 16, 64);
     s += __shfl_xor(s, 32, 64);
     const int lane = (int)(threadIdx.x & 63u), wid = (int)(threadIdx.x >> 6);
@@ -207,7 +237,7 @@ struct S2Slot {
 // One launch: steps (1 or 2) CG iterations from iteration k. in_parts: the producer's partial
 // sums (16 per block, block-major); status->bb holds ||b||^2. check_only: convergence test only.
 __global__ __launch_bounds__(kPcgThreads) void pcg_s2_kernel(Geo g, uint32_t per, PlaneTiling pt, const uint8_t* __restrict__ code,
-                                                             S2Vecs v, const double* __restrict__ in_parts, uint32_t in_blocks,
+                                                             S2Sets v, const double* __restrict__ in_parts, uint32_t in_blocks,
                                                              double* __restrict__ out_parts, PcgStatus* status, double tol2, uint32_t k,
                                                              uint32_t steps, uint32_t check_only) {
     __shared__ float lp[2][kTY + 4][kTX + 4];    // p'' of a plane, tile rim 2
@@ -216,10 +246,13 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_s2_kernel(Geo g, uint32_t per
     __shared__ float linv[8];                    // 1/d
     __shared__ double lds[16 * kMaxWaves];
     __shared__ double tot[16];
-    const uint32_t stop = status->stop;
+    // the producer's partials first: every later step of the launch waits for them. A launch that
+    // stops (or follows one that stopped) writes -inf into its rho partial, so the launches queued
+    // behind a stop return after this reduction without touching the vectors.
+    double pl[kS2PartLoads];
+    s2_load_partials(in_parts, in_blocks, pl);
     const uint32_t lb = xcd_block();
     if (threadIdx.x < 8) linv[threadIdx.x] = inv_diag(threadIdx.x);
-    if (stop) return;   // block-uniform, before any barrier
 
     int x0, y0, z0, z1;
     plane_block(g, pt, lb, x0, y0, z0, z1);
@@ -270,18 +303,23 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_s2_kernel(Geo g, uint32_t per
         const int w = ((zz % g.nz) + g.nz) % g.nz;
         return sz * (uint32_t)w;
     };
+    // buffer resources of the two sets, the increment and the code; vector m of a set at byte
+    // m * vstride
+    const uint32_t vstride = (n + 1u) * 4u;
+    const __amdgpu_buffer_rsrc_t rin = s2_resource(v.in, 5u * vstride), rout = s2_resource(v.out, 5u * vstride);
+    const __amdgpu_buffer_rsrc_t rxd = s2_resource(v.xd, vstride), rcode = s2_resource(code, n + 1u);
     auto load = [&](int t, S2Slot& S) {
         const uint32_t off = plane_off(t);
         const uint32_t io = min(col + off, n);
 #pragma unroll
-        for (int m = 0; m < 5; ++m) S.v[m] = v.v[m][io];
-        S.d = code[io];
-        S.xd = t >= z0 && t < z1 ? v.xd[io] : 0.0f;   // block-uniform
+        for (int m = 0; m < 5; ++m) S.v[m] = s2_load(rin, io * 4u, (uint32_t)m * vstride);
+        S.d = __builtin_amdgcn_raw_buffer_load_b8(rcode, (int)io, 0, 0);
+        S.xd = t >= z0 && t < z1 ? s2_load(rxd, io * 4u, 0u) : 0.0f;   // block-uniform
         if (rim) {
             const uint32_t ir = min(rcol + off, n);
 #pragma unroll
-            for (int m = 0; m < 5; ++m) S.rv[m] = v.v[m][ir];
-            S.rd = r1 ? code[ir] : (uint8_t)0;
+            for (int m = 0; m < 5; ++m) S.rv[m] = s2_load(rin, ir * 4u, (uint32_t)m * vstride);
+            S.rd = r1 ? __builtin_amdgcn_raw_buffer_load_b8(rcode, (int)ir, 0, 0) : (uint8_t)0;
         }
     };
 
@@ -292,7 +330,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_s2_kernel(Geo g, uint32_t per
         for (int j = 0; j < kS2Slots; ++j) load(t0 + j, slot[j]);
     }
 
-    s2_totals(in_parts, in_blocks, lds, tot);   // its barriers also publish linv
+    s2_totals(pl, lds, tot);   // its barriers also publish linv
     const double rho = tot[kS2Rho];
     const double bb = status->bb;
     double G[5][5];
@@ -309,10 +347,18 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_s2_kernel(Geo g, uint32_t per
     }
     const S2Coef cf = s2_coefficients(G, steps);
     if (!(rho > tol2 * bb) || check_only || !cf.ok) {   // converged (NaN also stops; the true residual decides)
+        const bool stopping = !(rho > tol2 * bb) || !cf.ok;
         if (lb == 0 && threadIdx.x == 0) {
-            status->rr = rho;
-            status->iterations = k;
-            if (!(rho > tol2 * bb) || !cf.ok) status->stop = 1u;
+            // the first launch that stops records the state; later ones only pass the marker on
+            if (!status->stop) {
+                status->rr = rho;
+                status->iterations = k;
+                if (stopping) status->stop = 1u;
+            }
+        }
+        if (stopping && threadIdx.x == 0) {
+#pragma unroll
+            for (int q = 0; q < 16; ++q) out_parts[16u * lb + q] = q == kS2Rho ? -INFINITY : 0.0;
         }
         return;
     }
@@ -350,7 +396,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_s2_kernel(Geo g, uint32_t per
             const float ia = linv[S.d & 7u];
             if (own && t >= z0 && t < z1) {
                 const float xn = __fmaf_rn(ce[3], S.v[3], __fmaf_rn(ce[1], S.v[1], __fmaf_rn(ce[0], S.v[0], S.xd)));
-                v.xd[col + plane_off(t)] = xn;
+                s2_store(xn, rxd, (col + plane_off(t)) * 4u, 0u);
             }
             lp[b0][oy][ox] = pa;
             lz[b1][ly + 1][lx + 1] = zw2;   // z'' of plane t-1, read by stage C of plane t-1
@@ -382,7 +428,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_s2_kernel(Geo g, uint32_t per
                 const uint32_t i = col + plane_off(tc);
                 const float nv[5]{pw0, qw1, P2, zw1, Z1};
 #pragma unroll
-                for (int m = 0; m < 5; ++m) v.o[m][i] = nv[m];
+                for (int m = 0; m < 5; ++m) s2_store(nv[m], rout, i * 4u, (uint32_t)m * vstride);
                 const double dd = (double)dw0;
                 double q[5], wq[5];
 #pragma unroll
