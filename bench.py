@@ -80,21 +80,32 @@ def cpu_baseline(grid, sim_steps, k, sample_steps):
 
 
 def calibrate_copy(mib):
-    """Plain device copies of a known size: FETCH_SIZE / WRITE_SIZE of these launches give the
-    counter-to-bytes factor that the PCG kernels' counters are scaled by."""
+    """Device copies of a known size, each launched 10 times: torch's copy (16-byte accesses per
+    lane) and the library's stream_copy_kernel (4-byte accesses per lane, as the PCG kernels).
+    FETCH_SIZE / WRITE_SIZE of these launches give the counter-to-bytes factor that the PCG
+    kernels' counters are scaled by."""
     import torch
+
+    from ds_smoke import abi, scenes
+    from ds_smoke.solver import Context
 
     n = mib * (1 << 20) // 4
     src = torch.ones(n, dtype=torch.float32, device="cuda")
     dst = torch.empty_like(src)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(10):
-        dst.copy_(src)
-    torch.cuda.synchronize()
-    dt = (time.perf_counter() - t0) / 10
-    print(json.dumps({"copy_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4,
-                      "ms_per_copy": dt * 1e3, "gbs": 2 * n * 4 / dt / 1e9}))
+    ctx = Context(scenes.plume(16).config(), abi.MODE_FORWARD)
+    out = {"copy_mib": mib, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4}
+    for name, copy in (("torch_copy", lambda: dst.copy_(src)),
+                       ("stream_copy_kernel", lambda: ctx.stream_copy(src, dst))):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(10):
+            copy()
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        dt = (time.perf_counter() - t0) / 10
+        out[name] = {"ms_per_copy": dt * 1e3, "gbs": 2 * n * 4 / dt / 1e9}
+    ctx.close()
+    print(json.dumps(out))
 
 
 # Multiply-adds per fine voxel of one inference of the warm-start network (3-level U-Net,

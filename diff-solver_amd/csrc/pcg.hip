@@ -328,6 +328,12 @@ __global__ void pcg_clear_stop_kernel(PcgStatus* status) {
     if (threadIdx.x == 0) status->stop = 0u;
 }
 
+// dst = src with one 4-byte load and one 4-byte store per lane and iteration, the access width of
+// the PCG iteration kernels.
+__global__ __launch_bounds__(256) void stream_copy_kernel(const float* __restrict__ src, float* __restrict__ dst, uint64_t count) {
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count; i += (uint64_t)gridDim.x * blockDim.x) dst[i] = src[i];
+}
+
 __global__ __launch_bounds__(256) void apply_matrix_kernel(Geo g, uint32_t per, const uint8_t* __restrict__ code,
                                                            const float* __restrict__ x, float* __restrict__ y) {
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
@@ -621,7 +627,8 @@ ds_solve_info pcg_iterate_s2(ds_context* c, float* xh, float* xl, double tol, ui
                                                                           respect_stop);
     };
     auto launch = [&](int set, uint32_t k, uint32_t steps, uint32_t check_only) {
-        pcg_s2_kernel<<<w.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, w.tiling, c->code, vecs(set), parts[set], w.blocks,
+        const S2Sets sets{w.s2set[set], w.s2set[set ^ 1], w.xd};
+        pcg_s2_kernel<<<w.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, w.tiling, c->code, sets, parts[set], w.blocks,
                                                                parts[set ^ 1], w.status, tol2, k, steps, check_only);
     };
     ds_solve_info info{};
@@ -1037,6 +1044,18 @@ ds_status ds_apply_pressure_matrix(ds_context* c, const float* x, float* y) {
         return DS_ERROR_INVALID_ARGUMENT;
     }
     return guard([&] { launch_apply_matrix(c, x, y); });
+}
+
+ds_status ds_stream_copy(ds_context* c, const float* src, float* dst, uint64_t count) {
+    if (!c || (count && (!src || !dst))) {
+        set_error("ds_stream_copy: null argument");
+        return DS_ERROR_INVALID_ARGUMENT;
+    }
+    return guard([&] {
+        if (count == 0) return;
+        stream_copy_kernel<<<2048, 256, 0, c->stream>>>(src, dst, count);
+        check_launch("stream_copy_kernel");
+    });
 }
 
 }  // extern "C"

@@ -75,6 +75,7 @@ def load_library(path: str | None = None):
         "ds_projection_vjp": (C.c_int32, [P, C.POINTER(P), C.POINTER(P), C.POINTER(abi.SolveInfo)]),
         "ds_pcg_solve": (C.c_int32, [P, P, P, P, C.c_double, C.c_uint32, C.POINTER(abi.SolveInfo)]),
         "ds_apply_pressure_matrix": (C.c_int32, [P, P, P]),
+        "ds_stream_copy": (C.c_int32, [P, P, P, C.c_uint64]),
         "ds_profile_enable": (C.c_int32, [P, C.c_uint32]),
         "ds_profile_read": (C.c_int32, [P, C.POINTER(abi.KernelTiming), C.c_uint32, C.POINTER(C.c_uint32)]),
         "ds_copy_state": (C.c_int32, [P, C.POINTER(abi.State), C.POINTER(abi.State)]),
@@ -591,6 +592,11 @@ class Context:
         return {out[i].name.decode(): (out[i].launches, out[i].samples,
                                        out[i].total_ms / out[i].samples if out[i].samples else float("nan"))
                 for i in range(n.value)}
+
+    def stream_copy(self, src, dst):
+        """dst = src (float32 device tensors of equal size) by 4-byte accesses per lane."""
+        assert src.numel() == dst.numel() and src.dtype == dst.dtype
+        self._check(self.lib.ds_stream_copy(self.h, _ptr(src), _ptr(dst), src.numel()), "ds_stream_copy")
 
     def apply_matrix(self, x):
         y = self.zeros_cell()

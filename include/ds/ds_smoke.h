@@ -367,6 +367,10 @@ ds_status ds_pcg_solve(ds_context* c, const float* b, float* x, const float* x0,
                        uint32_t max_iterations, ds_solve_info* info);
 /* y = A x on unknowns (0 elsewhere); for tests. */
 ds_status ds_apply_pressure_matrix(ds_context* c, const float* x, float* y);
+/* dst = src for count floats on the context's stream, one 4-byte load and store per lane (the
+ * access width of the PCG kernels): a launch of known traffic that scales the hardware's memory
+ * counters to bytes. No counterpart in the reference. */
+ds_status ds_stream_copy(ds_context* c, const float* src, float* dst, uint64_t count);
 
 /* ---- warm-start network (DS_WARM_START_CNN; no counterpart in the reference, whose projection
  * starts every solve from p = 0, operators.cu:1005) ------------------------------------------ */
