@@ -82,11 +82,13 @@ def test_gradient_matches_the_fp64_oracle(gradient128):
         w = np.unravel_index(int(np.argmax(d)), d.shape)
         print(f"  mirror plane axis {a}: ||ref|| {np.linalg.norm(rp):.3e} (axis {ref['axis_norms'][a]:.3e}), ||diff|| "
               f"{np.linalg.norm(d):.3e}, max |diff| {d.max():.3e} at (y, x) = {tuple(int(v) for v in w)} (ref {rp[w]:.4e})")
-    assert e_loss < 1e-5
-    assert e_par < 1e-4
+    # measured (both CG recurrences): loss 6.3e-8, parameters 1.6e-6, 32 DOFs 1.0e-4, projections
+    # 1.8e-7, axis norms 6.4e-6
+    assert e_loss < 1e-6
+    assert e_par < 2e-5
     assert e_dof < 1e-3
-    assert max(e_dots) < 1e-5
-    assert e_norm < 1e-4
+    assert max(e_dots) < 2e-6
+    assert e_norm < 6e-5
     # measured 1.43e-3 (x faces; 3.3e-4 and 3.8e-4 for y and z), the same under the one-step and
     # the s-step recurrences: 10 of the plane's entries hold 43% of the squared error (traces that
     # end near a sampling plane), the median entry is within 6e-4 of the FP64 value
