@@ -167,9 +167,15 @@ __device__ __forceinline__ Corner8 corners(const Geo& g, float px, float py, flo
     const float gx = px / g.h - ox, gy = py / g.h - oy, gz = pz / g.h - oz;
     Corner8 c;
     const float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
-    c.x0 = (int)fx;
-    c.y0 = (int)fy;
-    c.z0 = (int)fz;
+    // A non-finite or huge position (a diverged solve upstream) must not reach the index
+    // arithmetic: clamped to +-2^24 cells (NaN to the lower bound) before the conversion, the
+    // corner lies far outside the domain, fails every interior test without overflowing (x0 + 1
+    // stays representable) and takes the boundary-mapped loads; finite positions within 2^24
+    // cells of the domain are unchanged.
+    constexpr float kFar = 16777216.0f;
+    c.x0 = (int)fminf(fmaxf(fx, -kFar), kFar);
+    c.y0 = (int)fminf(fmaxf(fy, -kFar), kFar);
+    c.z0 = (int)fminf(fmaxf(fz, -kFar), kFar);
     c.tx = gx - fx;
     c.ty = gy - fy;
     c.tz = gz - fz;
@@ -339,9 +345,10 @@ __device__ __forceinline__ bool solid_at(float px, float py, float pz, const uin
     if (vperiodic(b, 0)) { const float L = g.nx * g.h; px -= floorf(px / L) * L; }
     if (vperiodic(b, 1)) { const float L = g.ny * g.h; py -= floorf(py / L) * L; }
     if (vperiodic(b, 2)) { const float L = g.nz * g.h; pz -= floorf(pz / L) * L; }
-    const int x = max(0, min(g.nx - 1, (int)floorf(px / g.h)));
-    const int y = max(0, min(g.ny - 1, (int)floorf(py / g.h)));
-    const int z = max(0, min(g.nz - 1, (int)floorf(pz / g.h)));
+    // clamped as floats, so a non-finite position maps to a cell of the domain (NaN to 0)
+    const int x = (int)fminf(fmaxf(floorf(px / g.h), 0.0f), (float)(g.nx - 1));
+    const int y = (int)fminf(fmaxf(floorf(py / g.h), 0.0f), (float)(g.ny - 1));
+    const int z = (int)fminf(fmaxf(floorf(pz / g.h), 0.0f), (float)(g.nz - 1));
     return mask[cidx(g, x, y, z)] != 0;
 }
 

@@ -117,6 +117,9 @@ __device__ __forceinline__ S2Coef s2_coefficients(const double (&G)[5][5], uint3
         k.a[i] = p[i];
         k.c[i] = z[i];
         k.e[i] = x[i];
+        // coefficients that do not fit FP32 would put non-finite values into the basis and the
+        // solution: stop instead, and let the true residual decide
+        k.ok = k.ok && fabs(p[i]) < 1e30 && fabs(z[i]) < 1e30 && fabs(x[i]) < 1e30;
     }
     return k;
 }
