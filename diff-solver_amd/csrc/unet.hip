@@ -401,30 +401,42 @@ const float* unet_infer(ds_context* c, const float* b, const float* xb) {
     residual_norm_kernel<<<dim3(kPcgBlocks, 1), kBlock, 0, s>>>((uint32_t)n1, rn, coden, u->norm_partials);
     pack_batch_input_kernel<<<dim3(kPcgBlocks, 1), kBlock, 0, s>>>((uint32_t)n1, u->net_unknowns, rn, coden, u->norm_partials,
                                                                    kPcgBlocks, u->in4);
-    constexpr int MT = 4;
-    auto conv_waves = [&](const Dims& d) { return wave_blocks((uint64_t)(d.nx / 16) * (d.ny / MT) * d.nz); };
     auto fwd = [&](int k, uint16_t* out) {
         ConvOut o;
         o.out = out;
         o.bias = u->bias[k];
         return o;
     };
+    // a 3x3x3 convolution with ReLU: 4 rows and every channel tile per wave where that gives the
+    // grid at least kConvWaves waves, else 1 row per wave, else 1 row and 1 channel tile per wave
+    // (the 32^3 and 16^3 levels of a 64^3 network)
+    constexpr uint64_t kConvWaves = 2048;
+    auto conv = [&]<int C1, int C2, int COUT>(const Dims& d, const uint16_t* in1, const uint16_t* in2, int k, uint16_t* out) {
+        const uint64_t units = (uint64_t)(d.nx / 16) * d.ny * d.nz;   // 16-voxel x rows
+        if (d.ny % 4 == 0 && units / 4 >= kConvWaves)
+            conv3_kernel<C1, C2, COUT, 4, kEpiBiasRelu><<<wave_blocks(units / 4), 256, 0, s>>>(d, 1, in1, in2, u->wpk[k], fwd(k, out));
+        else if (units >= kConvWaves || COUT == 16)
+            conv3_kernel<C1, C2, COUT, 1, kEpiBiasRelu><<<wave_blocks(units), 256, 0, s>>>(d, 1, in1, in2, u->wpk[k], fwd(k, out));
+        else
+            conv3_kernel<C1, C2, COUT, 1, kEpiBiasRelu, 0, 1><<<wave_blocks(units * (COUT / 16)), 256, 0, s>>>(d, 1, in1, in2, u->wpk[k],
+                                                                                                          fwd(k, out));
+    };
     // encoder
-    conv3_kernel<2, 0, 16, MT, kEpiBiasRelu><<<conv_waves(d1), 256, 0, s>>>(d1, 1, u->in4, nullptr, u->wpk[0], fwd(0, u->f1));
-    conv3_kernel<16, 0, 16, MT, kEpiBiasRelu><<<conv_waves(d1), 256, 0, s>>>(d1, 1, u->f1, nullptr, u->wpk[1], fwd(1, u->skip1));
+    conv.template operator()<2, 0, 16>(d1, u->in4, nullptr, 0, u->f1);
+    conv.template operator()<16, 0, 16>(d1, u->f1, nullptr, 1, u->skip1);
     maxpool2_kernel<16><<<blocks_for(n2 * 4), kBlock, 0, s>>>(d2, (uint32_t)n2, u->skip1, u->m1);
-    conv3_kernel<16, 0, 32, MT, kEpiBiasRelu><<<conv_waves(d2), 256, 0, s>>>(d2, 1, u->m1, nullptr, u->wpk[2], fwd(2, u->m2));
-    conv3_kernel<32, 0, 32, MT, kEpiBiasRelu><<<conv_waves(d2), 256, 0, s>>>(d2, 1, u->m2, nullptr, u->wpk[3], fwd(3, u->skip2));
+    conv.template operator()<16, 0, 32>(d2, u->m1, nullptr, 2, u->m2);
+    conv.template operator()<32, 0, 32>(d2, u->m2, nullptr, 3, u->skip2);
     maxpool2_kernel<32><<<blocks_for(n3 * 8), kBlock, 0, s>>>(d3, (uint32_t)n3, u->skip2, u->c1);
-    conv3_kernel<32, 0, 64, MT, kEpiBiasRelu><<<conv_waves(d3), 256, 0, s>>>(d3, 1, u->c1, nullptr, u->wpk[4], fwd(4, u->c2));
-    conv3_kernel<64, 0, 64, MT, kEpiBiasRelu><<<conv_waves(d3), 256, 0, s>>>(d3, 1, u->c2, nullptr, u->wpk[5], fwd(5, u->c1));
+    conv.template operator()<32, 0, 64>(d3, u->c1, nullptr, 4, u->c2);
+    conv.template operator()<64, 0, 64>(d3, u->c2, nullptr, 5, u->c1);
     // decoder
     tconv2_kernel<64, 32><<<wave_blocks((uint64_t)(d3.nx / 16) * d3.ny * d3.nz), 256, 0, s>>>(d3, 1, u->c1, u->wpk[6], u->bias[6], u->m1);
-    conv3_kernel<32, 32, 32, MT, kEpiBiasRelu><<<conv_waves(d2), 256, 0, s>>>(d2, 1, u->m1, u->skip2, u->wpk[7], fwd(7, u->m2));
-    conv3_kernel<32, 0, 32, MT, kEpiBiasRelu><<<conv_waves(d2), 256, 0, s>>>(d2, 1, u->m2, nullptr, u->wpk[8], fwd(8, u->m1));
+    conv.template operator()<32, 32, 32>(d2, u->m1, u->skip2, 7, u->m2);
+    conv.template operator()<32, 0, 32>(d2, u->m2, nullptr, 8, u->m1);
     tconv2_kernel<32, 16><<<wave_blocks((uint64_t)(d2.nx / 16) * d2.ny * d2.nz), 256, 0, s>>>(d2, 1, u->m1, u->wpk[9], u->bias[9], u->f1);
-    conv3_kernel<16, 16, 16, MT, kEpiBiasRelu><<<conv_waves(d1), 256, 0, s>>>(d1, 1, u->f1, u->skip1, u->wpk[10], fwd(10, u->f2));
-    conv3_kernel<16, 0, 16, MT, kEpiBiasRelu><<<conv_waves(d1), 256, 0, s>>>(d1, 1, u->f2, nullptr, u->wpk[11], fwd(11, u->f1));
+    conv.template operator()<16, 16, 16>(d1, u->f1, u->skip1, 10, u->f2);
+    conv.template operator()<16, 0, 16>(d1, u->f2, nullptr, 11, u->f1);
     head_kernel<<<blocks_for(n1), kBlock, 0, s>>>((uint32_t)n1, u->f1, u->head_w, u->bias[12], coden, L > 0 ? u->dc : u->dir);
     if (L > 0) prolong_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g, f, d1, c->code, u->dc, u->dir);
     step_partials_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g, per, c->code, r, u->dir, u->step_partials);

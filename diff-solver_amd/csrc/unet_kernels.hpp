@@ -81,38 +81,44 @@ struct ConvOut {
 
 // 3x3x3 convolution, stride 1, zero padding, over the channel concatenation [in1 (C1), in2 (C2)],
 // with the weights in fragment order wpk[((tap * NCH + ch) * NB + nb) * 64 + lane]. One wave
-// computes MT rows (y .. y+MT-1) of 16 voxels in x and all COUT channels; a block holds 4 waves.
-// C1 < 4 only without in2 (the input layer: in1 is stored with 4 channels, the rest load as 0).
-template <int C1, int C2, int COUT, int MT, int EPI, int CU = 0>
+// computes MT rows (y .. y+MT-1) of 16 voxels in x and NBW of the COUT / 16 channel tiles (all by
+// default; fewer, and one row, give the small grids of the coarse levels enough waves); a block
+// holds 4 waves. C1 < 4 only without in2 (the input layer: in1 is stored with 4 channels, the rest
+// load as 0).
+template <int C1, int C2, int COUT, int MT, int EPI, int CU = 0, int NBW = COUT / 16>
 __global__ __launch_bounds__(256) void conv3_kernel(Dims d, int batch, const uint16_t* __restrict__ in1,
                                                     const uint16_t* __restrict__ in2, const bf16x4_t* __restrict__ wpk, ConvOut o) {
     constexpr int NCH1 = (C1 + 15) / 16, NCH = NCH1 + C2 / 16, NB = COUT / 16;
+    constexpr int GROUPS = NB / NBW;
+    static_assert(NB % NBW == 0, "channel tiles per wave must divide the output tiles");
     constexpr int S1 = C1 < 4 ? 4 : C1;   // channel stride of in1
     const int lane = threadIdx.x & 63;
-    const uint32_t wave = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t wave_all = blockIdx.x * 4u + (threadIdx.x >> 6);
     const int tiles_x = d.nx / 16, rows = d.ny / MT;
-    const uint32_t total = (uint32_t)tiles_x * rows * d.nz * batch;
-    if (wave >= total) return;
+    const uint32_t total = (uint32_t)tiles_x * rows * d.nz * batch * GROUPS;
+    if (wave_all >= total) return;
+    const int nb0 = (int)(wave_all % (uint32_t)GROUPS) * NBW;
+    const uint32_t wave = wave_all / (uint32_t)GROUPS;
     const int tx = (int)(wave % (uint32_t)tiles_x);
     const int ry = (int)((wave / (uint32_t)tiles_x) % (uint32_t)rows);
     const int zg = (int)(wave / ((uint32_t)tiles_x * rows));
     const int zl = zg % d.nz;
     const int x0 = tx * 16, y0 = ry * MT;
     const int m = lane & 15, kq = lane >> 4;
-    floatx4_t acc[MT][NB];
+    floatx4_t acc[MT][NBW];
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) acc[t][nb] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+        for (int nb = 0; nb < NBW; ++nb) acc[t][nb] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
     for (int tap = 0; tap < 27; ++tap) {
         const int dz = tap / 9 - 1, dy = (tap / 3) % 3 - 1, dx = tap % 3 - 1;
         const int xx = x0 + m + dx;
         const bool okxz = xx >= 0 && xx < d.nx && zl + dz >= 0 && zl + dz < d.nz;
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch) {
-            bf16x4_t b[NB];
+            bf16x4_t b[NBW];
 #pragma unroll
-            for (int nb = 0; nb < NB; ++nb) b[nb] = wpk[(((size_t)tap * NCH + ch) * NB + nb) * 64 + lane];
+            for (int nb = 0; nb < NBW; ++nb) b[nb] = wpk[(((size_t)tap * NCH + ch) * NB + nb0 + nb) * 64 + lane];
             const bool first = ch < NCH1;
             const int cstride = first ? S1 : C2;
             const int cbase = (first ? ch * 16 : (ch - NCH1) * 16) + 4 * kq;
@@ -125,15 +131,15 @@ __global__ __launch_bounds__(256) void conv3_kernel(Dims d, int batch, const uin
                 if (okxz && okc && yy >= 0 && yy < d.ny)
                     a = *reinterpret_cast<const bf16x4_t*>(src + (size_t)vox(d, xx, yy, zg + dz) * cstride + cbase);
 #pragma unroll
-                for (int nb = 0; nb < NB; ++nb) acc[t][nb] = mfma(a, b[nb], acc[t][nb]);
+                for (int nb = 0; nb < NBW; ++nb) acc[t][nb] = mfma(a, b[nb], acc[t][nb]);
             }
         }
     }
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
-        for (int nb = 0; nb < NB; ++nb) {
-            const int co = nb * 16 + m;
+        for (int nb = 0; nb < NBW; ++nb) {
+            const int co = (nb0 + nb) * 16 + m;
             const float bv = (EPI == kEpiBias || EPI == kEpiBiasRelu) ? o.bias[co] : 0.0f;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
