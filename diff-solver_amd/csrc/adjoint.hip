@@ -28,13 +28,14 @@ struct VelSink {
 
 // Reverse of trace_rk2 + final sample (operators.cu:422-438 / 544-560): given the adjoint of the
 // trace end point (pbx, pby, pbz) = -deriv * dt * grad(sample) * out_adj, scatter through the
-// midpoint and start samples of the velocity.
+// midpoint and start samples of the velocity; mid: the velocity samples at the midpoint with their
+// gradients (from trace_rk2<.., true>).
 template <typename Sink>
-__device__ __forceinline__ void trace_reverse(const Geo& g, const VBound& b, const Vel3& u, const Sink& sink, float sx, float sy,
-                                              float sz, const TraceOut& t, float v1x, float v1y, float v1z) {
-    const Sample m0 = sample_face<0, true>(u.u[0], t.mx, t.my, t.mz, g, b);
-    const Sample m1 = sample_face<1, true>(u.u[1], t.mx, t.my, t.mz, g, b);
-    const Sample m2 = sample_face<2, true>(u.u[2], t.mx, t.my, t.mz, g, b);
+__device__ __forceinline__ void trace_reverse(const Geo& g, const VBound& b, const Sink& sink, float sx, float sy, float sz,
+                                              const TraceOut& t, const Sample (&mid)[3], float v1x, float v1y, float v1z) {
+    const Sample& m0 = mid[0];
+    const Sample& m1 = mid[1];
+    const Sample& m2 = mid[2];
     scatter_face_to<0>(sink, t.mx, t.my, t.mz, v1x, g, b);
     scatter_face_to<1>(sink, t.mx, t.my, t.mz, v1y, g, b);
     scatter_face_to<2>(sink, t.mx, t.my, t.mz, v1z, g, b);
@@ -67,7 +68,8 @@ __global__ __launch_bounds__(256) void advect_scalar_vjp_kernel(Geo g, const VBo
     int x, y, z;
     decode_cell(g, i, x, y, z);
     const float sx = (x + 0.5f) * g.h, sy = (y + 0.5f) * g.h, sz = (z + 0.5f) * g.h;
-    const TraceOut t = trace_rk2<HasSolids>(sx, sy, sz, u, mask, g, vb);
+    Sample mid[3];
+    const TraceOut t = trace_rk2<HasSolids, true>(sx, sy, sz, u, mask, g, vb, mid);
     const Sample s0 = sample_scalar<true>(src0, t.px, t.py, t.pz, g, sb0);
     scatter_scalar(src_adj0, t.px, t.py, t.pz, a0, g, sb0);
     float gx = s0.gx * a0, gy = s0.gy * a0, gz = s0.gz * a0;
@@ -79,7 +81,7 @@ __global__ __launch_bounds__(256) void advect_scalar_vjp_kernel(Geo g, const VBo
         gz += s1.gz * a1;
     }
     const float v1x = -(t.dx * g.dt) * gx, v1y = -(t.dy * g.dt) * gy, v1z = -(t.dz * g.dt) * gz;
-    trace_reverse(g, vb, u, VelSink{va}, sx, sy, sz, t, v1x, v1y, v1z);
+    trace_reverse(g, vb, VelSink{va}, sx, sy, sz, t, mid, v1x, v1y, v1z);
 }
 
 // advect_velocity_vjp_kernel (operators.cu:415-439) for the faces of axis A, with its scatters
@@ -139,11 +141,12 @@ __global__ __launch_bounds__(256) void advect_velocity_vjp_kernel(Geo g, const V
         const float sx = (x + (A == 0 ? 0.0f : 0.5f)) * g.h;
         const float sy = (y + (A == 1 ? 0.0f : 0.5f)) * g.h;
         const float sz = (z + (A == 2 ? 0.0f : 0.5f)) * g.h;
-        const TraceOut t = trace_rk2<HasSolids>(sx, sy, sz, u, mask, g, b);
+        Sample mid[3];
+        const TraceOut t = trace_rk2<HasSolids, true>(sx, sy, sz, u, mask, g, b, mid);
         const Sample s = sample_face<A, true>(u.u[A], t.px, t.py, t.pz, g, b);
         scatter_face_to<A>(sink, t.px, t.py, t.pz, a, g, b);
         const float v1x = -(t.dx * g.dt) * (s.gx * a), v1y = -(t.dy * g.dt) * (s.gy * a), v1z = -(t.dz * g.dt) * (s.gz * a);
-        trace_reverse(g, b, u, sink, sx, sy, sz, t, v1x, v1y, v1z);
+        trace_reverse(g, b, sink, sx, sy, sz, t, mid, v1x, v1y, v1z);
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 3 * kWin; i += blockDim.x) {

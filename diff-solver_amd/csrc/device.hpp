@@ -359,9 +359,11 @@ struct TraceOut {
 };
 
 // trace_rk2, operators.cu:256-276. has_solids = false skips the mask lookup (no colliders).
-template <bool HasSolids>
+// MidGrad: also return the three velocity samples at the midpoint with their gradients in mid
+// (the reverse and tangent sweeps need them; the values are the ones the trace uses).
+template <bool HasSolids, bool MidGrad = false>
 __device__ __forceinline__ TraceOut trace_rk2(float sx, float sy, float sz, const Vel3& u, const uint8_t* __restrict__ mask,
-                                              const Geo& g, const VBound& b) {
+                                              const Geo& g, const VBound& b, Sample* mid = nullptr) {
     TraceOut t;
     const float v0x = sample_face<0, false>(u.u[0], sx, sy, sz, g, b).value;
     const float v0y = sample_face<1, false>(u.u[1], sx, sy, sz, g, b).value;
@@ -369,9 +371,15 @@ __device__ __forceinline__ TraceOut trace_rk2(float sx, float sy, float sz, cons
     t.mx = sx - 0.5f * g.dt * v0x;
     t.my = sy - 0.5f * g.dt * v0y;
     t.mz = sz - 0.5f * g.dt * v0z;
-    const float v1x = sample_face<0, false>(u.u[0], t.mx, t.my, t.mz, g, b).value;
-    const float v1y = sample_face<1, false>(u.u[1], t.mx, t.my, t.mz, g, b).value;
-    const float v1z = sample_face<2, false>(u.u[2], t.mx, t.my, t.mz, g, b).value;
+    const Sample m0 = sample_face<0, MidGrad>(u.u[0], t.mx, t.my, t.mz, g, b);
+    const Sample m1 = sample_face<1, MidGrad>(u.u[1], t.mx, t.my, t.mz, g, b);
+    const Sample m2 = sample_face<2, MidGrad>(u.u[2], t.mx, t.my, t.mz, g, b);
+    if (MidGrad) {
+        mid[0] = m0;
+        mid[1] = m1;
+        mid[2] = m2;
+    }
+    const float v1x = m0.value, v1y = m1.value, v1z = m2.value;
     float rx = sx - g.dt * v1x, ry = sy - g.dt * v1y, rz = sz - g.dt * v1z;
     t.dx = t.dy = t.dz = 1.0f;
     const float Lx = g.nx * g.h, Ly = g.ny * g.h, Lz = g.nz * g.h;

@@ -129,14 +129,15 @@ template <bool HasSolids>
 __device__ __forceinline__ TraceOut trace_tangent(float sx, float sy, float sz, const Vel3& u, const Vel3& ut,
                                                   const uint8_t* __restrict__ mask, const Geo& g, const VBound& b, const VBound& bz,
                                                   float pos_t[3]) {
-    const TraceOut t = trace_rk2<HasSolids>(sx, sy, sz, u, mask, g, b);
+    Sample mid[3];
+    const TraceOut t = trace_rk2<HasSolids, true>(sx, sy, sz, u, mask, g, b, mid);
     const float hdt = -0.5f * g.dt;
     const float mtx = hdt * sample_face<0, false>(ut.u[0], sx, sy, sz, g, bz).value;
     const float mty = hdt * sample_face<1, false>(ut.u[1], sx, sy, sz, g, bz).value;
     const float mtz = hdt * sample_face<2, false>(ut.u[2], sx, sy, sz, g, bz).value;
-    const Sample s0 = sample_face<0, true>(u.u[0], t.mx, t.my, t.mz, g, b);
-    const Sample s1 = sample_face<1, true>(u.u[1], t.mx, t.my, t.mz, g, b);
-    const Sample s2 = sample_face<2, true>(u.u[2], t.mx, t.my, t.mz, g, b);
+    const Sample& s0 = mid[0];
+    const Sample& s1 = mid[1];
+    const Sample& s2 = mid[2];
     const float v1x = sample_face<0, false>(ut.u[0], t.mx, t.my, t.mz, g, bz).value + s0.gx * mtx + s0.gy * mty + s0.gz * mtz;
     const float v1y = sample_face<1, false>(ut.u[1], t.mx, t.my, t.mz, g, bz).value + s1.gx * mtx + s1.gy * mty + s1.gz * mtz;
     const float v1z = sample_face<2, false>(ut.u[2], t.mx, t.my, t.mz, g, bz).value + s2.gx * mtx + s2.gy * mty + s2.gz * mtz;
