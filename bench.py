@@ -43,6 +43,30 @@ BYTES_OTHER["pcg_resident_kernel"] = BYTES_OTHER["pcg_fused_kernel"]
 # pcg_s2_kernel (s-step form, two iterations per launch): every cell reads the code, the five
 # basis vectors and the solution increment and writes the new basis and the increment
 BYTES_UNKNOWN["pcg_s2_kernel"] = BYTES_OTHER["pcg_s2_kernel"] = 1 + 5 * 4 + 4 + 5 * 4 + 4
+# pcg_mg_iteration (multigrid-preconditioned form; the timed unit is one iteration: the V-cycle,
+# the direction and the update kernel). Level 0 per unknown: pre-smoothing reads code and r and
+# writes x (9), restriction reads code, r, x and writes an eighth of a coarse value (9.5), the first
+# post sweep reads code, r, x, an eighth of the coarse result and writes y (13.5), the second reads
+# code, r, y and writes z (13), the direction reads code, z, p_old and writes p (13), the update
+# reads code, p, r, x_d and writes r, x_d (21); other cells read the code and write the zeros of
+# x, y, z, p (22.5 with the restriction's share). Each coarser cell: diagonal, three couplings,
+# right-hand side and iterate per kernel (105 over its four kernels); the coarsest level's FP64
+# inverse is read once (mg_bytes adds both).
+BYTES_UNKNOWN["pcg_mg_iteration"] = 9 + 9.5 + 13.5 + 13 + 13 + 21
+BYTES_OTHER["pcg_mg_iteration"] = 5 + 1.5 + 5 + 5 + 5 + 1
+MG_COARSE_BYTES_PER_CELL = 105
+MG_COARSEST = 512
+
+
+def mg_bytes(res):
+    """Bytes of the coarser levels and the coarsest inverse per multigrid iteration: 2 x 2 x 2
+    aggregation (extents halved, rounded up) until at most MG_COARSEST cells."""
+    dims, coarse = list(res), 0
+    while dims[0] * dims[1] * dims[2] > MG_COARSEST:
+        dims = [(d + 1) // 2 for d in dims]
+        coarse += dims[0] * dims[1] * dims[2]
+    last = dims[0] * dims[1] * dims[2]
+    return MG_COARSE_BYTES_PER_CELL * (coarse - (last if coarse else 0)) + 4 * last * 2 + 8 * last * last
 
 
 def host_threads():
@@ -174,7 +198,7 @@ def main():
                          "network modes when its weights exist)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "diff-solver_amd", "data", "warm_start_plume.npz"),
                     help="warm-start network parameters (for --warm-start cnn)")
-    ap.add_argument("--cg-variant", choices=["single", "two", "resident", "s2"], default="s2",
+    ap.add_argument("--cg-variant", choices=["single", "two", "resident", "s2", "mg"], default="s2",
                     help="PCG recurrence: single-reduction (one fused kernel per iteration), resident (the same "
                          "recurrence as one cooperative launch per replacement interval where the grid allows), "
                          "two-reduction, or s2 (s-step: two iterations per fused launch)")
@@ -194,7 +218,7 @@ def main():
 
     sc = scenes.plume(args.grid)
     variant = {"single": abi.CG_SINGLE_REDUCTION, "two": abi.CG_TWO_REDUCTION,
-               "resident": abi.CG_SINGLE_REDUCTION_RESIDENT, "s2": abi.CG_S_STEP}[args.cg_variant]
+               "resident": abi.CG_SINGLE_REDUCTION_RESIDENT, "s2": abi.CG_S_STEP, "mg": abi.CG_MULTIGRID}[args.cg_variant]
     if args.warm_start == "all":
         modes = ["none", "extrapolate"] + (["cnn_alone", "cnn"] if os.path.exists(args.weights) else [])
     else:
@@ -246,7 +270,7 @@ def main():
         n, nf = ctx.domain.cell_count, ctx.domain.unknown_count
         kern = {}
         for name, (launches, samples, avg_ms) in prof.items():
-            b = BYTES_UNKNOWN[name] * nf + BYTES_OTHER[name] * (n - nf)
+            b = BYTES_UNKNOWN[name] * nf + BYTES_OTHER[name] * (n - nf) + (mg_bytes(sc.resolution) if name == "pcg_mg_iteration" else 0)
             kern[name] = {"launches": launches, "samples": samples, "avg_us": avg_ms * 1e3, "algorithmic_bytes": b,
                           "gbs": b / (avg_ms * 1e-3) / 1e9 if samples else None}
         ctx.close()

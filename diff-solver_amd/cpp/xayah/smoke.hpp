@@ -138,7 +138,8 @@ enum class CgVariant : std::uint32_t {
     single_reduction = DS_CG_SINGLE_REDUCTION,
     two_reduction = DS_CG_TWO_REDUCTION,
     single_reduction_resident = DS_CG_SINGLE_REDUCTION_RESIDENT,
-    s_step = DS_CG_S_STEP
+    s_step = DS_CG_S_STEP,
+    multigrid = DS_CG_MULTIGRID
 };
 
 struct Configuration {

@@ -103,7 +103,7 @@ void validate(const ds_config& cfg, uint32_t mode) {
     if ((cfg.warm_start == DS_WARM_START_CNN || cfg.warm_start == DS_WARM_START_CNN_ALONE) && (cfg.resolution[0] % 64 || cfg.resolution[1] % 16 || cfg.resolution[2] % 4))
         throw Failure{DS_ERROR_UNSUPPORTED, "the warm-start network needs nx % 64 == 0, ny % 16 == 0 and nz % 4 == 0"};
     if (!(cfg.cg_tolerance >= 0.0) || !std::isfinite(cfg.cg_tolerance)) bad("cg_tolerance must be >= 0 and finite");
-    if (cfg.cg_variant > DS_CG_S_STEP) bad("cg_variant out of range");
+    if (cfg.cg_variant > DS_CG_MULTIGRID) bad("cg_variant out of range");
 }
 
 // model.cpp:56-182 plus the pressure connectivity (operators.cu:852-873): solid mask from cell
@@ -415,7 +415,9 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
         c->pcg.p[1] = device_alloc<float>(c, n + 1);
         for (float* v : {c->pcg.r, c->pcg.p[0], c->pcg.p[1]}) fill_zero(c, v, n + 1);
         // the resident single-reduction variant runs the same recurrence
-        c->pcg.variant = cfg.cg_variant == DS_CG_TWO_REDUCTION || cfg.cg_variant == DS_CG_S_STEP ? cfg.cg_variant : DS_CG_SINGLE_REDUCTION;
+        c->pcg.variant = cfg.cg_variant == DS_CG_TWO_REDUCTION || cfg.cg_variant == DS_CG_S_STEP || cfg.cg_variant == DS_CG_MULTIGRID
+                              ? cfg.cg_variant
+                              : DS_CG_SINGLE_REDUCTION;
         if (c->pcg.variant == DS_CG_S_STEP) {
             // every basis vector carries the zero pad element n (read by out-of-domain rim loads);
             // a set is addressed with 32-bit byte offsets
@@ -465,6 +467,7 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
         c->pcg.hist = device_alloc<double>(c, 2 * ((size_t)c->max_iterations + 4));
         c->pcg.status = device_alloc<PcgStatus>(c, 1);
         if (cfg.cg_variant == DS_CG_SINGLE_REDUCTION_RESIDENT) pcg_resident_prepare(c);
+        if (cfg.cg_variant == DS_CG_MULTIGRID) mg_prepare(c, d.code);
         c->metric_partials = device_alloc<double>(c, 6 * (size_t)kMetricBlocks);
         void* hs = nullptr;
         DS_HIP(hipHostMalloc(&hs, 2 * sizeof(PcgStatus), hipHostMallocDefault));

@@ -97,7 +97,9 @@ struct PcgStatus {
 
 // HIP-event timing of sampled PCG kernel launches (kernel 0 = direction, 1 = update, 2 = the
 // fused single-reduction iteration).
-constexpr int kProfiledKernels = 5;   // 3 = the resident single-reduction launch (units: iterations), 4 = the s-step launch
+// 3 = the resident single-reduction launch (units: iterations), 4 = the s-step launch, 5 = one
+// multigrid-preconditioned iteration (V-cycle, direction and update)
+constexpr int kProfiledKernels = 6;
 struct Profiler {
     uint32_t every = 0;
     uint64_t launches[kProfiledKernels]{};
@@ -168,6 +170,30 @@ struct PcgWork {
     // vectors on chip (pcg_resident.hpp; DS_CG_SINGLE_REDUCTION_RESIDENT), when the grid allows it
     bool resident = false;
     uint32_t* resident_sync = nullptr;   // ResidentSync (2 words)
+};
+
+// DS_CG_MULTIGRID (pcg_mg.hip): one level of the aggregation hierarchy. Level 0 is the pressure
+// matrix itself (code: the diagonal count, -1 between neighbouring unknowns); a coarser level holds
+// its Galerkin operator as the diagonal and, per axis, the coupling to the + neighbour (a periodic
+// axis wraps). Cells with a zero diagonal are not unknowns of their level.
+struct MgLevel {
+    int nx, ny, nz;
+    uint32_t n;
+    FastDiv dx, dy;                 // cell decode: divide by nx, ny
+    const uint8_t* code;            // level 0
+    const float* diag;              // coarser levels
+    const float* wp[3];
+};
+constexpr int kMgMaxLevels = 32;
+constexpr uint32_t kMgCoarsest = 512;   // at most this many cells on the level solved directly
+struct MgWork {
+    int levels = 0;
+    MgLevel lv[kMgMaxLevels]{};
+    // per level: right-hand side (levels > 0), pre-smoothed iterate, iterate after the first post
+    // sweep, result (levels > 0)
+    float* vec[kMgMaxLevels][4]{};
+    double* dense = nullptr;        // inverse of the coarsest operator, n x n (symmetric)
+    float* z = nullptr;             // the preconditioned residual (n + 1, zero pad)
 };
 
 }  // namespace ds
@@ -260,6 +286,7 @@ struct ds_context {
     float* vort_mt = nullptr;         // magnitude tangent (differentiable mode)
 
     ds::PcgWork pcg;
+    ds::MgWork mg;                    // DS_CG_MULTIGRID hierarchy
     ds::Profiler profiler;
     ds::UNet* unet = nullptr;         // warm-start network, loaded by ds_warm_start_load
 
@@ -301,6 +328,14 @@ void window_allocate(ds_context* c, SolveWindow& w);
 // Decides whether the single-reduction solve runs resident (full 64 x kTY tiles, at most 8 planes
 // per block, cooperative launch possible with every block co-resident) and allocates its state.
 void pcg_resident_prepare(ds_context* c);
+// DS_CG_MULTIGRID (pcg_mg.hip): builds the hierarchy from the host copy of the code; then per
+// iteration z = M^-1 r (the V-cycle, <r, z> partials into pcg.partials), the direction kernel
+// (convergence test, p = z + beta p_old, <p, A p> partials) and the update kernel (solution pair
+// x = (xh, xl), residual, ||r||^2 partials).
+void mg_prepare(ds_context* c, const std::vector<uint8_t>& code);
+void mg_precondition(ds_context* c, const float* r, float* z);
+void mg_direction(ds_context* c, const float* z, double tol2, uint32_t k, bool first, bool check_only);
+void mg_update(ds_context* c, uint32_t k, float* xh, float* xl);
 // DS_PROJECTION_RBGS (rbgs.hip): the diagonal table, then the solve of a right-hand-side kind
 // (1 forward, 2 the unrolled reverse, 3 tangent) into xh with xl = 0.
 void rbgs_prepare(ds_context* c);

@@ -721,6 +721,95 @@ ds_solve_info pcg_iterate_s2(ds_context* c, float* xh, float* xl, double tol, ui
     return info;
 }
 
+// Multigrid-preconditioned iteration (DS_CG_MULTIGRID, pcg_mg.hip) from the state left by
+// launch_setup (r; BB, RR partials) until the true residual meets tol or max_it. Iteration k: the
+// V-cycle z = M^-1 r, the direction kernel (stopping test, p, <p, A p>), the update kernel (which
+// adds to the solution pair directly); the true residual is recomputed every replacement interval.
+ds_solve_info pcg_iterate_mg(ds_context* c, float* xh, float* xl, double tol, uint32_t max_it, bool warm, bool rejected) {
+    PcgWork& w = c->pcg;
+    const uint32_t per = periodic_bits(c);
+    const double tol2 = tol * tol;
+    const uint32_t R = c->replacement_interval;
+    float* z = c->mg.z;
+    auto true_residual = [&](uint32_t respect_stop) {
+        pcg_true_residual_kernel<<<w.blocks, kPcgThreads, 0, c->stream>>>(c->geo, per, c->code, w.b, xh, xl, w.r, w.partials, w.status,
+                                                                          respect_stop);
+    };
+    ds_solve_info info{};
+    info.warm_started = warm ? 1u : 0u;
+    info.guess_rejected = rejected ? 1u : 0u;
+    uint32_t k = 0;
+    bool first = true;
+    uint32_t restarts = 0;
+    Profiler& prof = c->profiler;
+    while (true) {
+        uint32_t batch = 1;
+        int slot = 0, pending = -1;
+        bool stop_seen = false;
+        while (k < max_it && !stop_seen) {
+            const uint32_t nb = std::min(batch, max_it - k);
+            for (uint32_t j = 0; j < nb; ++j, ++k) {
+                hipEvent_t e0 = nullptr, e1 = nullptr;
+                if (prof.every && k % prof.every == 0 && prof.used + 2 <= 6000) {
+                    e0 = prof.take();
+                    e1 = prof.take();
+                }
+                if (e0) DS_HIP(hipEventRecord(e0, c->stream));
+                mg_precondition(c, w.r, z);
+                mg_direction(c, z, tol2, k, first, false);
+                first = false;
+                mg_update(c, k, xh, xl);
+                if (e1) {
+                    DS_HIP(hipEventRecord(e1, c->stream));
+                    prof.recs.push_back({5, e0, e1});
+                }
+                if (prof.every) prof.launches[5]++;
+                if ((k + 1u) % R == 0u && k + 1u < max_it) true_residual(1u);
+            }
+            check_launch("pcg multigrid iteration");
+            DS_HIP(hipMemcpyAsync(&w.host_status[slot], w.status, sizeof(PcgStatus), hipMemcpyDeviceToHost, c->stream));
+            DS_HIP(hipEventRecord(w.batch_event[slot], c->stream));
+            if (pending >= 0) {
+                DS_HIP(hipEventSynchronize(w.batch_event[pending]));
+                if (w.host_status[pending].stop) stop_seen = true;
+            }
+            pending = slot;
+            slot ^= 1;
+            batch = std::min<uint32_t>(batch * 2u, 8u);
+        }
+        PcgStatus st = read_status(c);
+        if (!st.stop && k >= max_it) {
+            // one more stopping test on the last update's residual (its V-cycle gives the RZ
+            // partials the test reads but does not use)
+            mg_precondition(c, w.r, z);
+            mg_direction(c, z, tol2, k, first, true);
+            first = false;
+            check_launch("mg_direction_kernel");
+            st = read_status(c);
+        }
+        const uint32_t done = st.stop ? st.iterations : max_it;
+        k = done;
+        true_residual(0u);
+        pcg_finalize_kernel<<<1, kBlock, 0, c->stream>>>(w.partials, w.blocks, w.status, 0u);
+        check_launch("pcg_true_residual_kernel");
+        const PcgStatus fin = read_status(c);
+        info.iterations = done;
+        info.rhs_norm = std::sqrt(fin.bb);
+        info.final_residual = std::sqrt(std::max(0.0, fin.rr));
+        const bool finite = std::isfinite(fin.rr);
+        const bool ok = finite && fin.rr <= tol2 * fin.bb;
+        info.converged = ok ? 1u : 0u;
+        if (!finite) info.final_residual = fin.rr;
+        if (ok || !finite || !st.stop || done >= max_it || ++restarts > 8) break;
+        // the recursive residual met the tolerance but the true one (now in r) does not: continue
+        // from it with the previous direction discarded
+        fill_zero(c, w.p[(k + 1u) & 1u], c->geo.n);
+        pcg_clear_stop_kernel<<<1, 64, 0, c->stream>>>(w.status);
+        check_launch("pcg_clear_stop_kernel");
+    }
+    return info;
+}
+
 // Iterates from the state left by launch_setup until the true residual meets tol or max_it.
 ds_solve_info pcg_iterate(ds_context* c, float* xh, float* xl, double tol, uint32_t max_it, bool warm, bool rejected) {
     PcgWork& w = c->pcg;
@@ -889,6 +978,7 @@ ds_solve_info pcg_solve_kind(ds_context* c, int kind, const float* const u[3], c
     }
     ds_solve_info info = c->pcg.variant == DS_CG_SINGLE_REDUCTION ? pcg_iterate_fused(c, xh, xl, tol, max_it, warm, rejected)
                          : c->pcg.variant == DS_CG_S_STEP        ? pcg_iterate_s2(c, xh, xl, tol, max_it, warm, rejected)
+                         : c->pcg.variant == DS_CG_MULTIGRID     ? pcg_iterate_mg(c, xh, xl, tol, max_it, warm, rejected)
                                                                  : pcg_iterate(c, xh, xl, tol, max_it, warm, rejected);
     info.initial_residual = warm && !rejected ? std::sqrt(std::max(0.0, rr0)) : info.rhs_norm;
     (void)bb;
@@ -970,7 +1060,7 @@ ds_status ds_profile_read(ds_context* c, ds_kernel_timing* out, uint32_t capacit
         }
         // the resident launch is reported per iteration: launches and samples count iterations
         const char* names[kProfiledKernels]{"pcg_direction_kernel", "pcg_update_kernel", "pcg_fused_kernel", "pcg_resident_kernel",
-                                            "pcg_s2_kernel"};
+                                            "pcg_s2_kernel", "pcg_mg_iteration"};
         uint32_t n = 0;
         for (int k = 0; k < kProfiledKernels; ++k) {   // kernels that ran
             if (prof.launches[k] == 0 || n >= capacity) continue;

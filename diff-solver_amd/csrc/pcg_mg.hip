@@ -1,0 +1,608 @@
+// pcg_mg.hip — conjugate gradients preconditioned by an aggregation multigrid V-cycle
+// (DS_CG_MULTIGRID): the same pressure system as the Jacobi forms (the reference solves it with
+// Jacobi-preconditioned CG, solvers/smoke/operators.cu), to the same FP64 stopping test on the true
+// residual, in a few tens of iterations instead of about a thousand at 128^3.
+//
+// Hierarchy: each coarser level aggregates 2 x 2 x 2 cells of the finer one (an odd extent leaves
+// a last aggregate one cell wide; an extent of 1 stays 1). The prolongation P copies a coarse value
+// to the aggregate's cells and the restriction is its transpose (the cells' sum), so the coarse
+// operator A_c = P^T A P sums the fine couplings over the aggregates: a 7-point operator again, held
+// as its diagonal and the per-axis couplings to the + neighbour, built once per context on the host
+// from the connectivity code (periodic wraps included). The coarsest level (at most kMgCoarsest
+// cells) is solved exactly with its inverse (FP64 Cholesky on the host, applied in FP64).
+//
+// V-cycle (symmetric, so a valid CG preconditioner): per level two damped Jacobi sweeps from zero
+// (omega = 0.8), the residual restricted, the coarse correction scaled by kMgOver (piecewise-constant
+// aggregation under-corrects smooth errors; scaling it is the usual remedy), then two damped Jacobi
+// sweeps. The pre-smoothing kernel forms the first sweep's values at the neighbours on the fly, the
+// first post-smoothing kernel applies the prolonged correction on the fly, so a level costs four
+// launches. Parameters from an FP64 CPU model of the 128^3 plume's first solve: 13 iterations at
+// over-correction 1.5 (25 at 1.0) against 1015 for Jacobi-PCG, to 1e-6.
+//
+// The outer loop is textbook PCG with two reductions per iteration: the direction kernel forms
+// p = z + beta p_old and <p, A p>; the update kernel adds alpha p to the solution increment, updates
+// the residual and forms ||r||^2; the V-cycle's last kernel forms <r, z>. Every scalar comes from
+// FP64 per-block partials that each consumer block reduces in the same order. Every kernel returns at
+// once after the stopping test has fired, so the iterations queued behind it cost only launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "context.hpp"
+
+namespace ds {
+
+namespace {
+
+constexpr float kMgOmega = 0.8f;
+constexpr float kMgOver = 1.5f;
+constexpr uint32_t kMgThreads = 512;   // the kernels over contiguous ranges of level 0
+
+__device__ __forceinline__ void mg_coords(const MgLevel& L, uint32_t i, int& x, int& y, int& z) {
+    const uint32_t t = fdiv(i, L.dx);
+    x = (int)(i - t * (uint32_t)L.nx);
+    const uint32_t u = fdiv(t, L.dy);
+    y = (int)(t - u * (uint32_t)L.ny);
+    z = (int)u;
+}
+
+// The neighbours of cell i along axis a (c: i's coordinate on that axis): indices of the + and the
+// - neighbour, -1 outside a non-periodic axis (a periodic axis wraps; on an extent of 1 both are i).
+struct Nb {
+    int64_t up, dn;
+};
+__device__ __forceinline__ Nb mg_neighbours(const MgLevel& L, uint32_t per, int a, int c, uint32_t i) {
+    const int ext = a == 0 ? L.nx : a == 1 ? L.ny : L.nz;
+    const int64_t stride = a == 0 ? 1 : a == 1 ? (int64_t)L.nx : (int64_t)L.nx * L.ny;
+    const bool pa = per & (1u << a);
+    Nb nb;
+    nb.up = c + 1 < ext ? (int64_t)i + stride : (pa ? (int64_t)i - (int64_t)(ext - 1) * stride : -1);
+    nb.dn = c > 0 ? (int64_t)i - stride : (pa ? (int64_t)i + (int64_t)(ext - 1) * stride : -1);
+    return nb;
+}
+
+template <bool Fine>
+__device__ __forceinline__ float mg_diag(const MgLevel& L, uint32_t i) {
+    if constexpr (Fine) return (float)L.code[i];
+    else return L.diag[i];
+}
+
+// Off-diagonal part of (A v)_i at level L, v given by value(j) (0 off the level's unknowns). Level
+// 0: -1 per neighbouring unknown; coarser levels: i's + coupling and the - neighbour's + coupling.
+template <bool Fine, typename F>
+__device__ __forceinline__ float mg_offdiag(const MgLevel& L, uint32_t per, int x, int y, int z, uint32_t i, F&& value) {
+    const int c[3]{x, y, z};
+    float s = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const Nb nb = mg_neighbours(L, per, a, c[a], i);
+        if (nb.up >= 0) {
+            const float v = value((uint32_t)nb.up);
+            if constexpr (Fine) s -= v;
+            else s = __fmaf_rn(L.wp[a][i], v, s);
+        }
+        if (nb.dn >= 0) {
+            const float v = value((uint32_t)nb.dn);
+            if constexpr (Fine) s -= v;
+            else s = __fmaf_rn(L.wp[a][(uint32_t)nb.dn], v, s);
+        }
+    }
+    return s;
+}
+
+// Sum of value(j) over the level-0 neighbours of cell i, in FP64 (exact for FP32 values: A p with
+// the integer stencil carries no rounding into the residual recurrence).
+template <typename F>
+__device__ __forceinline__ double mg_neighbour_sum64(const MgLevel& L, uint32_t per, int x, int y, int z, uint32_t i, F&& value) {
+    const int c[3]{x, y, z};
+    double s = 0.0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const Nb nb = mg_neighbours(L, per, a, c[a], i);
+        if (nb.up >= 0) s += (double)value((uint32_t)nb.up);
+        if (nb.dn >= 0) s += (double)value((uint32_t)nb.dn);
+    }
+    return s;
+}
+
+// Contiguous cell range of this block (XCD-remapped) for the kernels that reduce over level 0.
+__device__ __forceinline__ void mg_range(uint32_t n, uint32_t& lb, uint32_t& start, uint32_t& end) {
+    lb = xcd_block();
+    uint32_t chunk = (n + gridDim.x - 1) / gridDim.x;
+    chunk = (chunk + blockDim.x - 1) / blockDim.x * blockDim.x;
+    start = min(n, lb * chunk);
+    end = min(n, start + chunk);
+}
+
+// Two damped Jacobi sweeps from zero for A x = r: x1 = w D^-1 r, x = x1 + w D^-1 (r - A x1), with
+// x1 at the neighbours formed on the fly.
+template <bool Fine>
+__global__ __launch_bounds__(kBlock) void mg_pre_kernel(MgLevel L, uint32_t per, const float* __restrict__ r, float* __restrict__ x,
+                                                        const PcgStatus* status) {
+    if (status->stop) return;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= L.n) return;
+    const float aii = mg_diag<Fine>(L, i);
+    if (!(aii > 0.0f)) {
+        x[i] = 0.0f;
+        return;
+    }
+    int cx, cy, cz;
+    mg_coords(L, i, cx, cy, cz);
+    auto x1 = [&](uint32_t j) {
+        const float ajj = mg_diag<Fine>(L, j);
+        return ajj > 0.0f ? kMgOmega * r[j] / ajj : 0.0f;
+    };
+    const float ri = r[i];
+    const float xi1 = kMgOmega * ri / aii;
+    const float ax1 = __fmaf_rn(aii, xi1, mg_offdiag<Fine>(L, per, cx, cy, cz, i, x1));
+    x[i] = xi1 + kMgOmega * (ri - ax1) / aii;
+}
+
+// rc_I = sum of (r - A x)_i over the cells i of aggregate I.
+template <bool Fine>
+__global__ __launch_bounds__(kBlock) void mg_restrict_kernel(MgLevel L, MgLevel C, uint32_t per, const float* __restrict__ r,
+                                                             const float* __restrict__ x, float* __restrict__ rc, const PcgStatus* status) {
+    if (status->stop) return;
+    const uint32_t I = xcd_block() * blockDim.x + threadIdx.x;
+    if (I >= C.n) return;
+    int X, Y, Z;
+    mg_coords(C, I, X, Y, Z);
+    float s = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int cx = 2 * X + (k & 1), cy = 2 * Y + ((k >> 1) & 1), cz = 2 * Z + (k >> 2);
+        if (cx >= L.nx || cy >= L.ny || cz >= L.nz) continue;
+        const uint32_t i = (uint32_t)cx + (uint32_t)L.nx * ((uint32_t)cy + (uint32_t)L.ny * (uint32_t)cz);
+        const float aii = mg_diag<Fine>(L, i);
+        if (!(aii > 0.0f)) continue;
+        const float ax = __fmaf_rn(aii, x[i], mg_offdiag<Fine>(L, per, cx, cy, cz, i, [&](uint32_t j) { return x[j]; }));
+        s += r[i] - ax;
+    }
+    rc[I] = s;
+}
+
+// The first post-smoothing sweep with the prolonged, over-corrected coarse correction applied on
+// the fly: xc = x + over P e, y = xc + w D^-1 (r - A xc).
+template <bool Fine>
+__global__ __launch_bounds__(kBlock) void mg_post1_kernel(MgLevel L, MgLevel C, uint32_t per, const float* __restrict__ r,
+                                                          const float* __restrict__ x, const float* __restrict__ e, float* __restrict__ y,
+                                                          const PcgStatus* status) {
+    if (status->stop) return;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= L.n) return;
+    const float aii = mg_diag<Fine>(L, i);
+    if (!(aii > 0.0f)) {
+        y[i] = 0.0f;
+        return;
+    }
+    int cx, cy, cz;
+    mg_coords(L, i, cx, cy, cz);
+    auto xc = [&](uint32_t j) {
+        if (!(mg_diag<Fine>(L, j) > 0.0f)) return 0.0f;
+        int jx, jy, jz;
+        mg_coords(L, j, jx, jy, jz);
+        const uint32_t J = (uint32_t)(jx >> 1) + (uint32_t)C.nx * ((uint32_t)(jy >> 1) + (uint32_t)C.ny * (uint32_t)(jz >> 1));
+        return __fmaf_rn(kMgOver, e[J], x[j]);
+    };
+    const float xi = xc(i);
+    const float axc = __fmaf_rn(aii, xi, mg_offdiag<Fine>(L, per, cx, cy, cz, i, xc));
+    y[i] = xi + kMgOmega * (r[i] - axc) / aii;
+}
+
+// One damped Jacobi sweep z = y + w D^-1 (r - A y) on a coarser level.
+__global__ __launch_bounds__(kBlock) void mg_post2_kernel(MgLevel L, uint32_t per, const float* __restrict__ r, const float* __restrict__ y,
+                                                          float* __restrict__ z, const PcgStatus* status) {
+    if (status->stop) return;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= L.n) return;
+    const float aii = L.diag[i];
+    if (!(aii > 0.0f)) {
+        z[i] = 0.0f;
+        return;
+    }
+    int cx, cy, cz;
+    mg_coords(L, i, cx, cy, cz);
+    const float yi = y[i];
+    const float ay = __fmaf_rn(aii, yi, mg_offdiag<false>(L, per, cx, cy, cz, i, [&](uint32_t j) { return y[j]; }));
+    z[i] = yi + kMgOmega * (r[i] - ay) / aii;
+}
+
+// The same sweep on level 0, over contiguous block ranges, with the partials of <r, z>.
+__global__ __launch_bounds__(kMgThreads) void mg_post2_fine_kernel(MgLevel L, uint32_t per, const float* __restrict__ r,
+                                                                   const float* __restrict__ y, float* __restrict__ z,
+                                                                   double* __restrict__ partials, const PcgStatus* status) {
+    __shared__ double lds[kMaxWaves];
+    if (status->stop) return;
+    uint32_t lb, start, end;
+    mg_range(L.n, lb, start, end);
+    double rz = 0.0;
+    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+        const uint32_t d = L.code[i];
+        if (d == 0u) {
+            z[i] = 0.0f;
+            continue;
+        }
+        const float aii = (float)d;
+        int cx, cy, cz;
+        mg_coords(L, i, cx, cy, cz);
+        const float yi = y[i], ri = r[i];
+        const float ay = __fmaf_rn(aii, yi, mg_offdiag<true>(L, per, cx, cy, cz, i, [&](uint32_t j) { return y[j]; }));
+        const float zi = yi + kMgOmega * (ri - ay) / aii;
+        z[i] = zi;
+        rz = __fma_rn((double)ri, (double)zi, rz);
+    }
+    double v[1]{rz};
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) partials[kPartRZ * kPcgMaxBlocks + lb] = v[0];
+}
+
+// Coarsest level: x = A^-1 r in FP64, one block per row of the inverse.
+__global__ __launch_bounds__(kBlock) void mg_dense_kernel(uint32_t n, const double* __restrict__ inv, const float* __restrict__ r,
+                                                          float* __restrict__ x, const PcgStatus* status) {
+    __shared__ double lds[kMaxWaves];
+    if (status->stop) return;
+    const uint32_t I = blockIdx.x;
+    double s = 0.0;
+    for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) s = __fma_rn(inv[(size_t)I * n + j], (double)r[j], s);
+    double v[1]{s};
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) x[I] = (float)v[0];
+}
+
+// Partials of <r, z> on level 0 (when level 0 is the coarsest and the dense solve produced z).
+__global__ __launch_bounds__(kMgThreads) void mg_dot_kernel(uint32_t n, const float* __restrict__ r, const float* __restrict__ z,
+                                                            double* __restrict__ partials, const PcgStatus* status) {
+    __shared__ double lds[kMaxWaves];
+    if (status->stop) return;
+    uint32_t lb, start, end;
+    mg_range(n, lb, start, end);
+    double rz = 0.0;
+    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) rz = __fma_rn((double)r[i], (double)z[i], rz);
+    double v[1]{rz};
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) partials[kPartRZ * kPcgMaxBlocks + lb] = v[0];
+}
+
+// PCG direction of iteration k: stopping test on ||r||^2 (RR partials) against tol^2 ||b||^2 (BB
+// partials on the first launch of a solve, then the status), beta = rz_k / rz_{k-1} (RZ partials of
+// the V-cycle), p = z + beta p_old (p = z when k = 0), partials of <p, A p>; A p is formed from p
+// at the neighbours on the fly.
+__global__ __launch_bounds__(kMgThreads) void mg_direction_kernel(MgLevel L, uint32_t per, const float* __restrict__ z,
+                                                                  const float* __restrict__ p_old, float* __restrict__ p_new,
+                                                                  double* __restrict__ partials, double* __restrict__ hist,
+                                                                  PcgStatus* status, double tol2, uint32_t k, uint32_t first,
+                                                                  uint32_t check_only) {
+    __shared__ double lds[3 * kMaxWaves];
+    if (status->stop) return;
+    const double* parts[3]{partials + kPartBB * kPcgMaxBlocks, partials + kPartRR * kPcgMaxBlocks, partials + kPartRZ * kPcgMaxBlocks};
+    double red[3];
+    reduce_partials<3>(parts, gridDim.x, red, lds);
+    const double bb = first ? red[0] : status->bb, rho = red[1], rz = red[2];
+    uint32_t lb, start, end;
+    mg_range(L.n, lb, start, end);
+    const bool converged = !(rho > tol2 * bb);
+    if (converged || check_only) {
+        if (lb == 0 && threadIdx.x == 0) {
+            if (first) status->bb = bb;
+            status->rr = rho;
+            status->iterations = k;
+            if (converged) status->stop = 1u;
+        }
+        return;
+    }
+    if (lb == 0 && threadIdx.x == 0) {
+        if (first) status->bb = bb;
+        hist[2 * k] = rz;
+    }
+    const bool with_old = k > 0;
+    const float beta = with_old ? (float)(rz / hist[2 * (k - 1)]) : 0.0f;
+    const uint8_t* __restrict__ code = L.code;
+    auto pn = [&](uint32_t j) -> float {
+        if (!code[j]) return 0.0f;
+        return with_old ? __fmaf_rn(beta, p_old[j], z[j]) : z[j];
+    };
+    double pq = 0.0;
+    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+        const uint32_t d = code[i];
+        if (d == 0u) {
+            p_new[i] = 0.0f;
+            continue;
+        }
+        int cx, cy, cz;
+        mg_coords(L, i, cx, cy, cz);
+        const float pi = pn(i);
+        p_new[i] = pi;
+        const double q = (double)d * (double)pi - mg_neighbour_sum64(L, per, cx, cy, cz, i, pn);
+        pq = __fma_rn((double)pi, q, pq);
+    }
+    double v[1]{pq};
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) partials[kPartPQ * kPcgMaxBlocks + lb] = v[0];
+}
+
+// PCG update of iteration k: alpha = rz_k / <p, A p>; x += alpha p; r -= alpha A p; partials of
+// ||r||^2. The solution pair (hi + lo, about 48 bits) takes alpha p in FP64 and A p is exact, so
+// the recursive residual follows the true one to FP32 rounding of r itself: with the near-null
+// mode of the matrix the solution is some 10^3 ||b|| large, and an FP32 increment (or an FP32
+// A p) would leave a gap of about 1e-4 ||b|| between them after the first, largest steps.
+__global__ __launch_bounds__(kMgThreads) void mg_update_kernel(MgLevel L, uint32_t per, const float* __restrict__ p, float* __restrict__ r,
+                                                               float* __restrict__ xh, float* __restrict__ xl, double* __restrict__ partials,
+                                                               const double* __restrict__ hist, const PcgStatus* status, uint32_t k) {
+    __shared__ double lds[kMaxWaves];
+    if (status->stop) return;
+    const double* parts[1]{partials + kPartPQ * kPcgMaxBlocks};
+    double red[1];
+    reduce_partials<1>(parts, gridDim.x, red, lds);
+    const double alpha = hist[2 * k] / red[0];
+    uint32_t lb, start, end;
+    mg_range(L.n, lb, start, end);
+    double rr = 0.0;
+    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
+        const uint32_t d = L.code[i];
+        if (d == 0u) continue;
+        int cx, cy, cz;
+        mg_coords(L, i, cx, cy, cz);
+        const float pi = p[i];
+        const double q = (double)d * (double)pi - mg_neighbour_sum64(L, per, cx, cy, cz, i, [&](uint32_t j) { return p[j]; });
+        const double xv = __fma_rn(alpha, (double)pi, (double)xh[i] + (double)xl[i]);
+        const float hi = (float)xv;
+        xh[i] = hi;
+        xl[i] = (float)(xv - (double)hi);
+        const float ri = (float)__fma_rn(-alpha, q, (double)r[i]);
+        r[i] = ri;
+        rr = __fma_rn((double)ri, (double)ri, rr);
+    }
+    double v[1]{rr};
+    block_sum<1>(v, lds);
+    if (threadIdx.x == 0) partials[kPartRR * kPcgMaxBlocks + lb] = v[0];
+}
+
+// ---- hierarchy (host) ------------------------------------------------------------------------
+
+struct HostLevel {
+    int nx = 0, ny = 0, nz = 0;
+    std::vector<double> diag;    // A_ii
+    std::vector<double> wp[3];   // coupling to the + neighbour per axis (wrapped on a periodic axis)
+    size_t cells() const { return (size_t)nx * ny * nz; }
+};
+
+// Index of the + neighbour of (x, y, z) along a, or SIZE_MAX outside a non-periodic axis.
+size_t plus_neighbour(const HostLevel& L, const bool per[3], int x, int y, int z, int a) {
+    int q[3]{x, y, z};
+    const int e[3]{L.nx, L.ny, L.nz};
+    if (++q[a] >= e[a]) {
+        if (!per[a]) return SIZE_MAX;
+        q[a] = 0;
+    }
+    return (size_t)q[0] + (size_t)L.nx * ((size_t)q[1] + (size_t)L.ny * (size_t)q[2]);
+}
+
+// Galerkin operator P^T A P of the 2 x 2 x 2 aggregation of level f. Each + edge (i, j) of weight w
+// adds 2 w to the aggregate's diagonal when i and j share an aggregate (the edge's two off-diagonal
+// entries), else w to the + coupling of i's aggregate along the edge's axis (a periodic wrap of the
+// fine level is a wrap of the coarse level too).
+HostLevel coarsen(const HostLevel& f, const bool per[3]) {
+    HostLevel c;
+    c.nx = (f.nx + 1) / 2;
+    c.ny = (f.ny + 1) / 2;
+    c.nz = (f.nz + 1) / 2;
+    const size_t nc = c.cells();
+    c.diag.assign(nc, 0.0);
+    for (auto& w : c.wp) w.assign(nc, 0.0);
+    auto agg = [&](size_t j) {
+        const size_t x = j % (size_t)f.nx, t = j / (size_t)f.nx;
+        const size_t y = t % (size_t)f.ny, z = t / (size_t)f.ny;
+        return x / 2 + (size_t)c.nx * (y / 2 + (size_t)c.ny * (z / 2));
+    };
+    for (int z = 0; z < f.nz; ++z)
+        for (int y = 0; y < f.ny; ++y)
+            for (int x = 0; x < f.nx; ++x) {
+                const size_t i = (size_t)x + (size_t)f.nx * ((size_t)y + (size_t)f.ny * (size_t)z);
+                const size_t I = (size_t)(x / 2) + (size_t)c.nx * ((size_t)(y / 2) + (size_t)c.ny * (size_t)(z / 2));
+                c.diag[I] += f.diag[i];
+                for (int a = 0; a < 3; ++a) {
+                    const double w = f.wp[a][i];
+                    if (w == 0.0) continue;
+                    const size_t j = plus_neighbour(f, per, x, y, z, a);
+                    if (j == SIZE_MAX) continue;
+                    if (agg(j) == I) c.diag[I] += 2.0 * w;
+                    else c.wp[a][I] += w;
+                }
+            }
+    return c;
+}
+
+// Inverse of level L's operator over its unknowns (diag > 0), FP64 Cholesky, as a symmetric
+// n x n matrix with zero rows and columns off the unknowns.
+std::vector<double> dense_inverse(const HostLevel& L, const bool per[3]) {
+    const size_t n = L.cells();
+    std::vector<int64_t> pos(n, -1);
+    std::vector<size_t> unk;
+    for (size_t i = 0; i < n; ++i)
+        if (L.diag[i] > 0.0) {
+            pos[i] = (int64_t)unk.size();
+            unk.push_back(i);
+        }
+    const size_t m = unk.size();
+    std::vector<double> A(m * m, 0.0);
+    for (int z = 0; z < L.nz; ++z)
+        for (int y = 0; y < L.ny; ++y)
+            for (int x = 0; x < L.nx; ++x) {
+                const size_t i = (size_t)x + (size_t)L.nx * ((size_t)y + (size_t)L.ny * (size_t)z);
+                if (pos[i] < 0) continue;
+                const size_t pi = (size_t)pos[i];
+                A[pi * m + pi] += L.diag[i];
+                for (int a = 0; a < 3; ++a) {
+                    const double w = L.wp[a][i];
+                    if (w == 0.0) continue;
+                    const size_t j = plus_neighbour(L, per, x, y, z, a);
+                    if (j == SIZE_MAX || pos[j] < 0) continue;
+                    const size_t pj = (size_t)pos[j];
+                    A[pi * m + pj] += w;
+                    A[pj * m + pi] += w;
+                }
+            }
+    // A = G G^T, G lower triangular (in place)
+    for (size_t j = 0; j < m; ++j) {
+        double s = A[j * m + j];
+        for (size_t k = 0; k < j; ++k) s -= A[j * m + k] * A[j * m + k];
+        if (!(s > 0.0)) throw Failure{DS_ERROR_RUNTIME, "multigrid: the coarsest operator is not positive definite"};
+        const double d = std::sqrt(s);
+        A[j * m + j] = d;
+        for (size_t i = j + 1; i < m; ++i) {
+            double t = A[i * m + j];
+            for (size_t k = 0; k < j; ++k) t -= A[i * m + k] * A[j * m + k];
+            A[i * m + j] = t / d;
+        }
+    }
+    // column c of A^-1: G y = e_c, then G^T x = y
+    std::vector<double> inv(m * m);
+    std::vector<double> col(m);
+    for (size_t c = 0; c < m; ++c) {
+        for (size_t i = 0; i < m; ++i) {
+            double t = i == c ? 1.0 : 0.0;
+            for (size_t k = 0; k < i; ++k) t -= A[i * m + k] * col[k];
+            col[i] = t / A[i * m + i];
+        }
+        for (size_t i = m; i-- > 0;) {
+            double t = col[i];
+            for (size_t k = i + 1; k < m; ++k) t -= A[k * m + i] * col[k];
+            col[i] = t / A[i * m + i];
+        }
+        for (size_t i = 0; i < m; ++i) inv[i * m + c] = col[i];
+    }
+    std::vector<double> out(n * n, 0.0);
+    for (size_t a = 0; a < m; ++a)
+        for (size_t b = 0; b < m; ++b) out[unk[a] * n + unk[b]] = 0.5 * (inv[a * m + b] + inv[b * m + a]);
+    return out;
+}
+
+uint32_t periodic_mask(const ds_context* c) {
+    return (c->pressure_periodic[0] ? 1u : 0u) | (c->pressure_periodic[1] ? 2u : 0u) | (c->pressure_periodic[2] ? 4u : 0u);
+}
+
+}  // namespace
+
+void mg_prepare(ds_context* c, const std::vector<uint8_t>& code) {
+    MgWork& w = c->mg;
+    const Geo& g = c->geo;
+    const bool per[3]{c->pressure_periodic[0], c->pressure_periodic[1], c->pressure_periodic[2]};
+    require(code.size() >= (size_t)g.n, "multigrid: code shorter than the grid");
+    // level 0 as diagonal + couplings: A_ii = code_i, -1 between neighbouring unknowns (on an
+    // extent-1 periodic axis a cell neighbours itself twice, as in the matrix-vector product)
+    std::vector<HostLevel> levels(1);
+    {
+        HostLevel& f = levels[0];
+        f.nx = g.nx;
+        f.ny = g.ny;
+        f.nz = g.nz;
+        const size_t n = f.cells();
+        f.diag.resize(n);
+        for (auto& v : f.wp) v.assign(n, 0.0);
+        for (int z = 0; z < g.nz; ++z)
+            for (int y = 0; y < g.ny; ++y)
+                for (int x = 0; x < g.nx; ++x) {
+                    const size_t i = (size_t)x + (size_t)g.nx * ((size_t)y + (size_t)g.ny * (size_t)z);
+                    f.diag[i] = (double)code[i];
+                    if (!code[i]) continue;
+                    for (int a = 0; a < 3; ++a) {
+                        const size_t j = plus_neighbour(f, per, x, y, z, a);
+                        if (j != SIZE_MAX && code[j]) f.wp[a][i] = -1.0;
+                    }
+                }
+    }
+    while (levels.back().cells() > kMgCoarsest) {
+        if ((int)levels.size() == kMgMaxLevels) throw Failure{DS_ERROR_UNSUPPORTED, "multigrid: too many levels"};
+        HostLevel next = coarsen(levels.back(), per);
+        levels.push_back(std::move(next));
+    }
+    w.levels = (int)levels.size();
+    for (int l = 0; l < w.levels; ++l) {
+        const HostLevel& H = levels[l];
+        MgLevel& L = w.lv[l];
+        L.nx = H.nx;
+        L.ny = H.ny;
+        L.nz = H.nz;
+        L.n = (uint32_t)H.cells();
+        L.dx = make_fastdiv((uint32_t)H.nx);
+        L.dy = make_fastdiv((uint32_t)H.ny);
+        L.code = l == 0 ? c->code : nullptr;
+        L.diag = nullptr;
+        for (int a = 0; a < 3; ++a) L.wp[a] = nullptr;
+        if (l > 0) {
+            std::vector<float> buf(L.n);
+            auto put = [&](const std::vector<double>& v) {
+                for (uint32_t i = 0; i < L.n; ++i) buf[i] = (float)v[i];
+                float* d = device_alloc<float>(c, L.n);
+                DS_HIP(hipMemcpy(d, buf.data(), L.n * sizeof(float), hipMemcpyHostToDevice));
+                return d;
+            };
+            L.diag = put(H.diag);
+            for (int a = 0; a < 3; ++a) L.wp[a] = put(H.wp[a]);
+            for (int k = 0; k < 4; ++k) w.vec[l][k] = device_alloc<float>(c, L.n);
+        } else {
+            for (int k = 1; k < 3; ++k) w.vec[l][k] = device_alloc<float>(c, L.n);
+        }
+        for (int k = 0; k < 4; ++k) fill_zero(c, w.vec[l][k], L.n);
+    }
+    const std::vector<double> inv = dense_inverse(levels.back(), per);
+    w.dense = device_alloc<double>(c, inv.size());
+    DS_HIP(hipMemcpy(w.dense, inv.data(), inv.size() * sizeof(double), hipMemcpyHostToDevice));
+    w.z = device_alloc<float>(c, (size_t)g.n + 1);
+    fill_zero(c, w.z, (size_t)g.n + 1);
+}
+
+// z = M^-1 r, the V-cycle from level 0 (r: level 0's right-hand side), with the partials of <r, z>.
+void mg_precondition(ds_context* c, const float* r, float* z) {
+    MgWork& w = c->mg;
+    PcgWork& pw = c->pcg;
+    const uint32_t per = periodic_mask(c);
+    const int Lc = w.levels - 1;
+    hipStream_t s = c->stream;
+    const PcgStatus* st = pw.status;
+    auto rhs = [&](int l) -> const float* { return l == 0 ? r : w.vec[l][0]; };
+    for (int l = 0; l < Lc; ++l) {
+        const MgLevel& L = w.lv[l];
+        const MgLevel& C = w.lv[l + 1];
+        if (l == 0) {
+            mg_pre_kernel<true><<<xcd_blocks_for(L.n), kBlock, 0, s>>>(L, per, r, w.vec[0][1], st);
+            mg_restrict_kernel<true><<<xcd_blocks_for(C.n), kBlock, 0, s>>>(L, C, per, r, w.vec[0][1], w.vec[1][0], st);
+        } else {
+            mg_pre_kernel<false><<<xcd_blocks_for(L.n), kBlock, 0, s>>>(L, per, rhs(l), w.vec[l][1], st);
+            mg_restrict_kernel<false><<<xcd_blocks_for(C.n), kBlock, 0, s>>>(L, C, per, rhs(l), w.vec[l][1], w.vec[l + 1][0], st);
+        }
+    }
+    mg_dense_kernel<<<w.lv[Lc].n, kBlock, 0, s>>>(w.lv[Lc].n, w.dense, rhs(Lc), Lc == 0 ? z : w.vec[Lc][3], st);
+    for (int l = Lc - 1; l >= 0; --l) {
+        const MgLevel& L = w.lv[l];
+        const MgLevel& C = w.lv[l + 1];
+        if (l == 0) {
+            mg_post1_kernel<true><<<xcd_blocks_for(L.n), kBlock, 0, s>>>(L, C, per, r, w.vec[0][1], w.vec[1][3], w.vec[0][2], st);
+            mg_post2_fine_kernel<<<pw.blocks, kMgThreads, 0, s>>>(L, per, r, w.vec[0][2], z, pw.partials, st);
+        } else {
+            mg_post1_kernel<false><<<xcd_blocks_for(L.n), kBlock, 0, s>>>(L, C, per, rhs(l), w.vec[l][1], w.vec[l + 1][3], w.vec[l][2], st);
+            mg_post2_kernel<<<xcd_blocks_for(L.n), kBlock, 0, s>>>(L, per, rhs(l), w.vec[l][2], w.vec[l][3], st);
+        }
+    }
+    if (Lc == 0) mg_dot_kernel<<<pw.blocks, kMgThreads, 0, s>>>(w.lv[0].n, r, z, pw.partials, st);
+    check_launch("multigrid V-cycle");
+}
+
+void mg_direction(ds_context* c, const float* z, double tol2, uint32_t k, bool first, bool check_only) {
+    PcgWork& w = c->pcg;
+    mg_direction_kernel<<<w.blocks, kMgThreads, 0, c->stream>>>(c->mg.lv[0], periodic_mask(c), z, w.p[(k + 1u) & 1u], w.p[k & 1u],
+                                                               w.partials, w.hist, w.status, tol2, k, first ? 1u : 0u,
+                                                               check_only ? 1u : 0u);
+}
+
+void mg_update(ds_context* c, uint32_t k, float* xh, float* xl) {
+    PcgWork& w = c->pcg;
+    mg_update_kernel<<<w.blocks, kMgThreads, 0, c->stream>>>(c->mg.lv[0], periodic_mask(c), w.p[k & 1u], w.r, xh, xl, w.partials, w.hist,
+                                                            w.status, k);
+}
+
+}  // namespace ds

@@ -22,7 +22,7 @@ SHAPE_ELLIPSOID, SHAPE_BOX = 0, 1
 PROJECTION_PCG, PROJECTION_RBGS = 0, 1
 WARM_START_NONE, WARM_START_PREVIOUS, WARM_START_CNN, WARM_START_EXTRAPOLATE, WARM_START_PROJECTION, WARM_START_CNN_ALONE = 0, 1, 2, 3, 4, 5
 MODE_FORWARD, MODE_DIFFERENTIABLE = 0, 1
-CG_SINGLE_REDUCTION, CG_TWO_REDUCTION, CG_SINGLE_REDUCTION_RESIDENT, CG_S_STEP = 0, 1, 2, 3
+CG_SINGLE_REDUCTION, CG_TWO_REDUCTION, CG_SINGLE_REDUCTION_RESIDENT, CG_S_STEP, CG_MULTIGRID = 0, 1, 2, 3, 4
 
 FloatP = C.POINTER(C.c_float)
 DoubleP = C.POINTER(C.c_double)

@@ -96,7 +96,12 @@ enum { DS_PROJECTION_PCG = 0, DS_PROJECTION_RBGS = 1 };
  * D-weighted Gram matrix gives both iterations' scalars; about 1.7x the one-step form's iterations
  * per second at 128^3). The stopping test runs every second iteration, so iteration counts are
  * even except at an odd iteration cap. */
-enum { DS_CG_SINGLE_REDUCTION = 0, DS_CG_TWO_REDUCTION = 1, DS_CG_SINGLE_REDUCTION_RESIDENT = 2, DS_CG_S_STEP = 3 };
+/* DS_CG_MULTIGRID: CG preconditioned by an aggregation multigrid V-cycle instead of Jacobi (2 x 2 x 2
+ * aggregates, Galerkin coarse operators built at context creation, two damped Jacobi sweeps before
+ * and after each coarse correction, the coarsest level of at most 512 cells solved exactly). Same
+ * stopping test on the true residual; a few tens of iterations where the Jacobi forms need about a
+ * thousand (128^3 plume). No counterpart in the reference, which iterates Jacobi-PCG. */
+enum { DS_CG_SINGLE_REDUCTION = 0, DS_CG_TWO_REDUCTION = 1, DS_CG_SINGLE_REDUCTION_RESIDENT = 2, DS_CG_S_STEP = 3, DS_CG_MULTIGRID = 4 };
 
 /* Initial guess of the forward and adjoint pressure solves. */
 /* DS_WARM_START_EXTRAPOLATE: each forward pressure solve of a sweep starts from the quadratic
