@@ -182,6 +182,7 @@ struct MgLevel {
     FastDiv dx, dy;                 // cell decode: divide by nx, ny
     const uint8_t* code;            // level 0
     const float* diag;              // coarser levels
+    const float* idiag;             // 1 / diag on the unknowns, 0 elsewhere (coarser levels)
     const float* wp[3];
 };
 constexpr int kMgMaxLevels = 32;
@@ -189,8 +190,8 @@ constexpr uint32_t kMgCoarsest = 512;   // at most this many cells on the level 
 struct MgWork {
     int levels = 0;
     MgLevel lv[kMgMaxLevels]{};
-    // per level: right-hand side (levels > 0), pre-smoothed iterate, iterate after the first post
-    // sweep, result (levels > 0)
+    // per level: right-hand side (levels > 0), pre-smoothed iterate (level 0), iterate after the
+    // first post sweep (level 0), result (levels > 0)
     float* vec[kMgMaxLevels][4]{};
     double* dense = nullptr;        // inverse of the coarsest operator, n x n (symmetric)
     float* z = nullptr;             // the preconditioned residual (n + 1, zero pad)

@@ -11,13 +11,14 @@
 // from the connectivity code (periodic wraps included). The coarsest level (at most kMgCoarsest
 // cells) is solved exactly with its inverse (FP64 Cholesky on the host, applied in FP64).
 //
-// V-cycle (symmetric, so a valid CG preconditioner): per level two damped Jacobi sweeps from zero
-// (omega = 0.8), the residual restricted, the coarse correction scaled by kMgOver (piecewise-constant
-// aggregation under-corrects smooth errors; scaling it is the usual remedy), then two damped Jacobi
-// sweeps. The pre-smoothing kernel forms the first sweep's values at the neighbours on the fly, the
-// first post-smoothing kernel applies the prolonged correction on the fly, so a level costs four
-// launches. Parameters from an FP64 CPU model of the 128^3 plume's first solve: 13 iterations at
-// over-correction 1.5 (25 at 1.0) against 1015 for Jacobi-PCG, to 1e-6.
+// V-cycle (symmetric, so a valid CG preconditioner): damped Jacobi sweeps from zero (two on level 0,
+// one on the coarser levels), the residual restricted, the coarse correction scaled up (piecewise-
+// constant aggregation under-corrects smooth errors; scaling it is the usual remedy), then as many
+// damped Jacobi sweeps again. Values a sweep needs at the neighbours are formed on the fly: level 0
+// costs four launches (pre-smoothing, restriction, two post sweeps), a coarser level two (down: the
+// restriction of the residual after its pre-smoothing; up: prolongation plus its post sweep).
+// Without the over-correction an FP64 CPU model of the 128^3 plume needs about twice the CG
+// iterations (25 against 13 on V(2,2)); Jacobi-PCG needs 1015.
 //
 // The outer loop is textbook PCG with two reductions per iteration: the direction kernel forms
 // p = z + beta p_old and <p, A p>; the update kernel adds alpha p to the solution increment, updates
@@ -37,8 +38,14 @@ namespace ds {
 
 namespace {
 
-constexpr float kMgOmega = 0.8f;
-constexpr float kMgOver = 1.5f;
+// Level 0 runs V(2,2) (two damped Jacobi sweeps before and after the coarse correction), the coarser
+// levels V(1,1): an FP64 CPU model of the 128^3 plume with random right-hand sides needs 14 CG
+// iterations to 1e-6 for these parameters, 13 with V(2,2) on every level (four launches per level
+// instead of two), 17 with V(1,1) everywhere.
+constexpr float kOmega0 = 0.9f;   // level-0 Jacobi damping
+constexpr float kOver0 = 1.8f;    // scale of the coarse correction prolonged to level 0
+constexpr float kOmegaC = 0.9f;   // coarser levels
+constexpr float kOverC = 1.5f;
 constexpr uint32_t kMgThreads = 512;   // the kernels over contiguous ranges of level 0
 
 __device__ __forceinline__ void mg_coords(const MgLevel& L, uint32_t i, int& x, int& y, int& z) {
@@ -49,63 +56,68 @@ __device__ __forceinline__ void mg_coords(const MgLevel& L, uint32_t i, int& x, 
     z = (int)u;
 }
 
-// The neighbours of cell i along axis a (c: i's coordinate on that axis): indices of the + and the
-// - neighbour, -1 outside a non-periodic axis (a periodic axis wraps; on an extent of 1 both are i).
-struct Nb {
-    int64_t up, dn;
-};
-__device__ __forceinline__ Nb mg_neighbours(const MgLevel& L, uint32_t per, int a, int c, uint32_t i) {
-    const int ext = a == 0 ? L.nx : a == 1 ? L.ny : L.nz;
-    const int64_t stride = a == 0 ? 1 : a == 1 ? (int64_t)L.nx : (int64_t)L.nx * L.ny;
-    const bool pa = per & (1u << a);
-    Nb nb;
-    nb.up = c + 1 < ext ? (int64_t)i + stride : (pa ? (int64_t)i - (int64_t)(ext - 1) * stride : -1);
-    nb.dn = c > 0 ? (int64_t)i - stride : (pa ? (int64_t)i + (int64_t)(ext - 1) * stride : -1);
-    return nb;
+// 1 / d for the level-0 diagonal counts 1..6, 0 off the unknowns (a chain of selects).
+__device__ __forceinline__ float mg_inv_count(uint32_t d) {
+    float v = 0.0f;
+    v = d == 1u ? 1.0f : v;
+    v = d == 2u ? 0.5f : v;
+    v = d == 3u ? (1.0f / 3.0f) : v;
+    v = d == 4u ? 0.25f : v;
+    v = d == 5u ? 0.2f : v;
+    v = d == 6u ? (1.0f / 6.0f) : v;
+    return v;
 }
 
-template <bool Fine>
-__device__ __forceinline__ float mg_diag(const MgLevel& L, uint32_t i) {
-    if constexpr (Fine) return (float)L.code[i];
-    else return L.diag[i];
+// Calls f(j, q, w) for every neighbour j of cell i = (x, y, z) on level L: q its coordinates, w the
+// coupling A_ij (level 0: -1, to be applied to values that are 0 off the unknowns; coarser levels:
+// i's + coupling, or the - neighbour's own). A periodic axis wraps (on an extent of 1 both
+// neighbours are i itself, as in the level-0 matrix-vector product); a non-periodic one ends.
+template <bool Fine, typename F>
+__device__ __forceinline__ void mg_for_neighbours(const MgLevel& L, uint32_t per, int x, int y, int z, uint32_t i, F&& f) {
+    const int ext[3]{L.nx, L.ny, L.nz};
+    const uint32_t stride[3]{1u, (uint32_t)L.nx, (uint32_t)L.nx * (uint32_t)L.ny};
+    const int c[3]{x, y, z};
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const bool pa = per & (1u << a);
+        if (c[a] + 1 < ext[a] || pa) {
+            int q[3]{x, y, z};
+            q[a] = c[a] + 1 < ext[a] ? c[a] + 1 : 0;
+            const uint32_t j = i + (uint32_t)(q[a] - c[a]) * stride[a];
+            f(j, q, Fine ? -1.0f : L.wp[a][i]);
+        }
+        if (c[a] > 0 || pa) {
+            int q[3]{x, y, z};
+            q[a] = c[a] > 0 ? c[a] - 1 : ext[a] - 1;
+            const uint32_t j = i + (uint32_t)(q[a] - c[a]) * stride[a];
+            f(j, q, Fine ? -1.0f : L.wp[a][j]);
+        }
+    }
 }
 
-// Off-diagonal part of (A v)_i at level L, v given by value(j) (0 off the level's unknowns). Level
-// 0: -1 per neighbouring unknown; coarser levels: i's + coupling and the - neighbour's + coupling.
+// Off-diagonal part of (A v)_i, v given by value(j, q) (0 off the level's unknowns).
 template <bool Fine, typename F>
 __device__ __forceinline__ float mg_offdiag(const MgLevel& L, uint32_t per, int x, int y, int z, uint32_t i, F&& value) {
-    const int c[3]{x, y, z};
     float s = 0.0f;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const Nb nb = mg_neighbours(L, per, a, c[a], i);
-        if (nb.up >= 0) {
-            const float v = value((uint32_t)nb.up);
-            if constexpr (Fine) s -= v;
-            else s = __fmaf_rn(L.wp[a][i], v, s);
-        }
-        if (nb.dn >= 0) {
-            const float v = value((uint32_t)nb.dn);
-            if constexpr (Fine) s -= v;
-            else s = __fmaf_rn(L.wp[a][(uint32_t)nb.dn], v, s);
-        }
-    }
+    mg_for_neighbours<Fine>(L, per, x, y, z, i, [&](uint32_t j, const int (&q)[3], float w) {
+        if constexpr (Fine) s -= value(j, q);
+        else s = __fmaf_rn(w, value(j, q), s);
+    });
     return s;
 }
 
-// Sum of value(j) over the level-0 neighbours of cell i, in FP64 (exact for FP32 values: A p with
-// the integer stencil carries no rounding into the residual recurrence).
+// The same on level 0 in FP64 (exact for FP32 values: A p with the integer stencil carries no
+// rounding into the residual recurrence).
 template <typename F>
-__device__ __forceinline__ double mg_neighbour_sum64(const MgLevel& L, uint32_t per, int x, int y, int z, uint32_t i, F&& value) {
-    const int c[3]{x, y, z};
+__device__ __forceinline__ double mg_offdiag64(const MgLevel& L, uint32_t per, int x, int y, int z, uint32_t i, F&& value) {
     double s = 0.0;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const Nb nb = mg_neighbours(L, per, a, c[a], i);
-        if (nb.up >= 0) s += (double)value((uint32_t)nb.up);
-        if (nb.dn >= 0) s += (double)value((uint32_t)nb.dn);
-    }
+    mg_for_neighbours<true>(L, per, x, y, z, i, [&](uint32_t j, const int (&)[3], float) { s -= (double)value(j); });
     return s;
+}
+
+// Index on level C of the aggregate holding the finer cell with coordinates q.
+__device__ __forceinline__ uint32_t mg_parent(const MgLevel& C, const int (&q)[3]) {
+    return (uint32_t)(q[0] >> 1) + (uint32_t)C.nx * ((uint32_t)(q[1] >> 1) + (uint32_t)C.ny * (uint32_t)(q[2] >> 1));
 }
 
 // Contiguous cell range of this block (XCD-remapped) for the kernels that reduce over level 0.
@@ -117,85 +129,99 @@ __device__ __forceinline__ void mg_range(uint32_t n, uint32_t& lb, uint32_t& sta
     end = min(n, start + chunk);
 }
 
-// Two damped Jacobi sweeps from zero for A x = r: x1 = w D^-1 r, x = x1 + w D^-1 (r - A x1), with
-// x1 at the neighbours formed on the fly.
-template <bool Fine>
-__global__ __launch_bounds__(kBlock) void mg_pre_kernel(MgLevel L, uint32_t per, const float* __restrict__ r, float* __restrict__ x,
-                                                        const PcgStatus* status) {
+// Level 0, two damped Jacobi sweeps from zero for A x = r: x1 = w D^-1 r, x = x1 + w D^-1 (r - A x1),
+// with x1 at the neighbours formed on the fly.
+__global__ __launch_bounds__(kBlock) void mg_pre0_kernel(MgLevel L, uint32_t per, const float* __restrict__ r, float* __restrict__ x,
+                                                         const PcgStatus* status) {
     if (status->stop) return;
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= L.n) return;
-    const float aii = mg_diag<Fine>(L, i);
-    if (!(aii > 0.0f)) {
+    const uint32_t d = L.code[i];
+    if (d == 0u) {
         x[i] = 0.0f;
         return;
     }
     int cx, cy, cz;
     mg_coords(L, i, cx, cy, cz);
-    auto x1 = [&](uint32_t j) {
-        const float ajj = mg_diag<Fine>(L, j);
-        return ajj > 0.0f ? kMgOmega * r[j] / ajj : 0.0f;
-    };
-    const float ri = r[i];
-    const float xi1 = kMgOmega * ri / aii;
-    const float ax1 = __fmaf_rn(aii, xi1, mg_offdiag<Fine>(L, per, cx, cy, cz, i, x1));
-    x[i] = xi1 + kMgOmega * (ri - ax1) / aii;
+    auto x1 = [&](uint32_t j, const int (&)[3]) { return kOmega0 * mg_inv_count(L.code[j]) * r[j]; };
+    const float ri = r[i], inv = mg_inv_count(d);
+    const float xi1 = kOmega0 * inv * ri;
+    const float ax1 = __fmaf_rn((float)d, xi1, mg_offdiag<true>(L, per, cx, cy, cz, i, x1));
+    x[i] = __fmaf_rn(kOmega0 * inv, ri - ax1, xi1);
 }
 
-// rc_I = sum of (r - A x)_i over the cells i of aggregate I.
-template <bool Fine>
-__global__ __launch_bounds__(kBlock) void mg_restrict_kernel(MgLevel L, MgLevel C, uint32_t per, const float* __restrict__ r,
-                                                             const float* __restrict__ x, float* __restrict__ rc, const PcgStatus* status) {
-    if (status->stop) return;
-    const uint32_t I = xcd_block() * blockDim.x + threadIdx.x;
-    if (I >= C.n) return;
+// Restriction with one thread per cell: thread t handles child k = t mod 8 of aggregate I = t / 8
+// (k = 1 x-bit + 2 y-bit + 4 z-bit), so the 8 lanes of an aligned group share I and their values are
+// summed with three butterfly shuffles (the same association in every lane). Sets the child's cell
+// coordinates and index; false when the child lies beyond an odd extent (it contributes 0).
+__device__ __forceinline__ bool mg_child(const MgLevel& L, const MgLevel& C, uint32_t t, uint32_t& I, int (&q)[3], uint32_t& i) {
+    I = t >> 3;
+    const uint32_t k = t & 7u;
     int X, Y, Z;
     mg_coords(C, I, X, Y, Z);
-    float s = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 8; ++k) {
-        const int cx = 2 * X + (k & 1), cy = 2 * Y + ((k >> 1) & 1), cz = 2 * Z + (k >> 2);
-        if (cx >= L.nx || cy >= L.ny || cz >= L.nz) continue;
-        const uint32_t i = (uint32_t)cx + (uint32_t)L.nx * ((uint32_t)cy + (uint32_t)L.ny * (uint32_t)cz);
-        const float aii = mg_diag<Fine>(L, i);
-        if (!(aii > 0.0f)) continue;
-        const float ax = __fmaf_rn(aii, x[i], mg_offdiag<Fine>(L, per, cx, cy, cz, i, [&](uint32_t j) { return x[j]; }));
-        s += r[i] - ax;
-    }
-    rc[I] = s;
+    q[0] = 2 * X + (int)(k & 1u);
+    q[1] = 2 * Y + (int)((k >> 1) & 1u);
+    q[2] = 2 * Z + (int)(k >> 2);
+    i = (uint32_t)q[0] + (uint32_t)L.nx * ((uint32_t)q[1] + (uint32_t)L.ny * (uint32_t)q[2]);
+    return q[0] < L.nx && q[1] < L.ny && q[2] < L.nz;
 }
 
-// The first post-smoothing sweep with the prolonged, over-corrected coarse correction applied on
-// the fly: xc = x + over P e, y = xc + w D^-1 (r - A xc).
-template <bool Fine>
-__global__ __launch_bounds__(kBlock) void mg_post1_kernel(MgLevel L, MgLevel C, uint32_t per, const float* __restrict__ r,
-                                                          const float* __restrict__ x, const float* __restrict__ e, float* __restrict__ y,
-                                                          const PcgStatus* status) {
+__device__ __forceinline__ float mg_group8_sum(float v) {
+    v += __shfl_xor(v, 1, 64);
+    v += __shfl_xor(v, 2, 64);
+    v += __shfl_xor(v, 4, 64);
+    return v;
+}
+
+// Level 0 to 1: rc_I = sum of (r - A x)_i over the cells i of aggregate I (x: the pre-smoothed
+// iterate). Grid: 8 threads per aggregate.
+__global__ __launch_bounds__(kBlock) void mg_restrict0_kernel(MgLevel L, MgLevel C, uint32_t per, const float* __restrict__ r,
+                                                              const float* __restrict__ x, float* __restrict__ rc, const PcgStatus* status) {
     if (status->stop) return;
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= L.n) return;
-    const float aii = mg_diag<Fine>(L, i);
-    if (!(aii > 0.0f)) {
-        y[i] = 0.0f;
-        return;
+    const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
+    if ((t >> 3) >= C.n) return;   // whole groups of 8 leave together
+    uint32_t I, i;
+    int q[3];
+    float v = 0.0f;
+    if (mg_child(L, C, t, I, q, i)) {
+        const uint32_t d = L.code[i];
+        if (d != 0u) {
+            const float ax =
+                __fmaf_rn((float)d, x[i], mg_offdiag<true>(L, per, q[0], q[1], q[2], i, [&](uint32_t j, const int (&)[3]) { return x[j]; }));
+            v = r[i] - ax;
+        }
     }
-    int cx, cy, cz;
-    mg_coords(L, i, cx, cy, cz);
-    auto xc = [&](uint32_t j) {
-        if (!(mg_diag<Fine>(L, j) > 0.0f)) return 0.0f;
-        int jx, jy, jz;
-        mg_coords(L, j, jx, jy, jz);
-        const uint32_t J = (uint32_t)(jx >> 1) + (uint32_t)C.nx * ((uint32_t)(jy >> 1) + (uint32_t)C.ny * (uint32_t)(jz >> 1));
-        return __fmaf_rn(kMgOver, e[J], x[j]);
-    };
-    const float xi = xc(i);
-    const float axc = __fmaf_rn(aii, xi, mg_offdiag<Fine>(L, per, cx, cy, cz, i, xc));
-    y[i] = xi + kMgOmega * (r[i] - axc) / aii;
+    v = mg_group8_sum(v);
+    if ((t & 7u) == 0u) rc[I] = v;
 }
 
-// One damped Jacobi sweep z = y + w D^-1 (r - A y) on a coarser level.
-__global__ __launch_bounds__(kBlock) void mg_post2_kernel(MgLevel L, uint32_t per, const float* __restrict__ r, const float* __restrict__ y,
-                                                          float* __restrict__ z, const PcgStatus* status) {
+// A coarser level l >= 1 on the way down (V(1,1)): the pre-smoothing x1 = w D^-1 r is formed on the
+// fly, so one launch gives rc_I = sum of (r - A x1)_i over the cells i of aggregate I on level l + 1.
+// Grid: 8 threads per aggregate.
+__global__ __launch_bounds__(kBlock) void mg_down_kernel(MgLevel L, MgLevel C, uint32_t per, const float* __restrict__ r,
+                                                         float* __restrict__ rc, const PcgStatus* status) {
+    if (status->stop) return;
+    const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
+    if ((t >> 3) >= C.n) return;   // whole groups of 8 leave together
+    uint32_t I, i;
+    int q[3];
+    float v = 0.0f;
+    if (mg_child(L, C, t, I, q, i)) {
+        const float aii = L.diag[i];
+        if (aii > 0.0f) {
+            auto x1 = [&](uint32_t j, const int (&)[3]) { return kOmegaC * L.idiag[j] * r[j]; };
+            const float ax = __fmaf_rn(aii, x1(i, q), mg_offdiag<false>(L, per, q[0], q[1], q[2], i, x1));
+            v = r[i] - ax;
+        }
+    }
+    v = mg_group8_sum(v);
+    if ((t & 7u) == 0u) rc[I] = v;
+}
+
+// A coarser level l >= 1 on the way up: xc = w D^-1 r + over P e (the pre-smoothed iterate plus the
+// prolonged correction, both formed on the fly), then one sweep z = xc + w D^-1 (r - A xc).
+__global__ __launch_bounds__(kBlock) void mg_up_kernel(MgLevel L, MgLevel C, uint32_t per, const float* __restrict__ r,
+                                                       const float* __restrict__ e, float* __restrict__ z, const PcgStatus* status) {
     if (status->stop) return;
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= L.n) return;
@@ -206,13 +232,41 @@ __global__ __launch_bounds__(kBlock) void mg_post2_kernel(MgLevel L, uint32_t pe
     }
     int cx, cy, cz;
     mg_coords(L, i, cx, cy, cz);
-    const float yi = y[i];
-    const float ay = __fmaf_rn(aii, yi, mg_offdiag<false>(L, per, cx, cy, cz, i, [&](uint32_t j) { return y[j]; }));
-    z[i] = yi + kMgOmega * (r[i] - ay) / aii;
+    auto xc = [&](uint32_t j, const int (&q)[3]) {
+        const float inv = L.idiag[j];
+        return inv > 0.0f ? __fmaf_rn(kOverC, e[mg_parent(C, q)], kOmegaC * inv * r[j]) : 0.0f;
+    };
+    const int c[3]{cx, cy, cz};
+    const float xi = xc(i, c);
+    const float axc = __fmaf_rn(aii, xi, mg_offdiag<false>(L, per, cx, cy, cz, i, xc));
+    z[i] = __fmaf_rn(kOmegaC * L.idiag[i], r[i] - axc, xi);
 }
 
-// The same sweep on level 0, over contiguous block ranges, with the partials of <r, z>.
-__global__ __launch_bounds__(kMgThreads) void mg_post2_fine_kernel(MgLevel L, uint32_t per, const float* __restrict__ r,
+// Level 0 on the way up, first sweep, with the prolonged correction applied on the fly:
+// xc = x + over P e, y = xc + w D^-1 (r - A xc).
+__global__ __launch_bounds__(kBlock) void mg_post0_kernel(MgLevel L, MgLevel C, uint32_t per, const float* __restrict__ r,
+                                                          const float* __restrict__ x, const float* __restrict__ e, float* __restrict__ y,
+                                                          const PcgStatus* status) {
+    if (status->stop) return;
+    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
+    if (i >= L.n) return;
+    const uint32_t d = L.code[i];
+    if (d == 0u) {
+        y[i] = 0.0f;
+        return;
+    }
+    int cx, cy, cz;
+    mg_coords(L, i, cx, cy, cz);
+    auto xc = [&](uint32_t j, const int (&q)[3]) { return L.code[j] ? __fmaf_rn(kOver0, e[mg_parent(C, q)], x[j]) : 0.0f; };
+    const int c[3]{cx, cy, cz};
+    const float xi = xc(i, c);
+    const float axc = __fmaf_rn((float)d, xi, mg_offdiag<true>(L, per, cx, cy, cz, i, xc));
+    y[i] = __fmaf_rn(kOmega0 * mg_inv_count(d), r[i] - axc, xi);
+}
+
+// Level 0, second sweep z = y + w D^-1 (r - A y), over contiguous block ranges, with the partials of
+// <r, z>.
+__global__ __launch_bounds__(kMgThreads) void mg_post0_last_kernel(MgLevel L, uint32_t per, const float* __restrict__ r,
                                                                    const float* __restrict__ y, float* __restrict__ z,
                                                                    double* __restrict__ partials, const PcgStatus* status) {
     __shared__ double lds[kMaxWaves];
@@ -226,12 +280,11 @@ __global__ __launch_bounds__(kMgThreads) void mg_post2_fine_kernel(MgLevel L, ui
             z[i] = 0.0f;
             continue;
         }
-        const float aii = (float)d;
         int cx, cy, cz;
         mg_coords(L, i, cx, cy, cz);
         const float yi = y[i], ri = r[i];
-        const float ay = __fmaf_rn(aii, yi, mg_offdiag<true>(L, per, cx, cy, cz, i, [&](uint32_t j) { return y[j]; }));
-        const float zi = yi + kMgOmega * (ri - ay) / aii;
+        const float ay = __fmaf_rn((float)d, yi, mg_offdiag<true>(L, per, cx, cy, cz, i, [&](uint32_t j, const int (&)[3]) { return y[j]; }));
+        const float zi = __fmaf_rn(kOmega0 * mg_inv_count(d), ri - ay, yi);
         z[i] = zi;
         rz = __fma_rn((double)ri, (double)zi, rz);
     }
@@ -301,10 +354,8 @@ __global__ __launch_bounds__(kMgThreads) void mg_direction_kernel(MgLevel L, uin
     const bool with_old = k > 0;
     const float beta = with_old ? (float)(rz / hist[2 * (k - 1)]) : 0.0f;
     const uint8_t* __restrict__ code = L.code;
-    auto pn = [&](uint32_t j) -> float {
-        if (!code[j]) return 0.0f;
-        return with_old ? __fmaf_rn(beta, p_old[j], z[j]) : z[j];
-    };
+    // z and p_old are 0 off the unknowns, so p_new is too
+    auto pn = [&](uint32_t j) -> float { return with_old ? __fmaf_rn(beta, p_old[j], z[j]) : z[j]; };
     double pq = 0.0;
     for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
         const uint32_t d = code[i];
@@ -316,7 +367,8 @@ __global__ __launch_bounds__(kMgThreads) void mg_direction_kernel(MgLevel L, uin
         mg_coords(L, i, cx, cy, cz);
         const float pi = pn(i);
         p_new[i] = pi;
-        const double q = (double)d * (double)pi - mg_neighbour_sum64(L, per, cx, cy, cz, i, pn);
+        // A p as the update forms it (exactly), so alpha and the residual recurrence agree
+        const double q = (double)d * (double)pi + mg_offdiag64(L, per, cx, cy, cz, i, pn);
         pq = __fma_rn((double)pi, q, pq);
     }
     double v[1]{pq};
@@ -347,7 +399,7 @@ __global__ __launch_bounds__(kMgThreads) void mg_update_kernel(MgLevel L, uint32
         int cx, cy, cz;
         mg_coords(L, i, cx, cy, cz);
         const float pi = p[i];
-        const double q = (double)d * (double)pi - mg_neighbour_sum64(L, per, cx, cy, cz, i, [&](uint32_t j) { return p[j]; });
+        const double q = (double)d * (double)pi + mg_offdiag64(L, per, cx, cy, cz, i, [&](uint32_t j) { return p[j]; });
         const double xv = __fma_rn(alpha, (double)pi, (double)xh[i] + (double)xl[i]);
         const float hi = (float)xv;
         xh[i] = hi;
@@ -532,6 +584,7 @@ void mg_prepare(ds_context* c, const std::vector<uint8_t>& code) {
         L.dy = make_fastdiv((uint32_t)H.ny);
         L.code = l == 0 ? c->code : nullptr;
         L.diag = nullptr;
+        L.idiag = nullptr;
         for (int a = 0; a < 3; ++a) L.wp[a] = nullptr;
         if (l > 0) {
             std::vector<float> buf(L.n);
@@ -542,10 +595,17 @@ void mg_prepare(ds_context* c, const std::vector<uint8_t>& code) {
                 return d;
             };
             L.diag = put(H.diag);
+            std::vector<double> inv(L.n);
+            for (uint32_t i = 0; i < L.n; ++i) inv[i] = H.diag[i] > 0.0 ? 1.0 / H.diag[i] : 0.0;
+            L.idiag = put(inv);
             for (int a = 0; a < 3; ++a) L.wp[a] = put(H.wp[a]);
-            for (int k = 0; k < 4; ++k) w.vec[l][k] = device_alloc<float>(c, L.n);
+            // right-hand side and result
+            w.vec[l][0] = device_alloc<float>(c, L.n);
+            w.vec[l][3] = device_alloc<float>(c, L.n);
         } else {
-            for (int k = 1; k < 3; ++k) w.vec[l][k] = device_alloc<float>(c, L.n);
+            // the pre-smoothed iterate and the iterate after the first post sweep
+            w.vec[l][1] = device_alloc<float>(c, L.n);
+            w.vec[l][2] = device_alloc<float>(c, L.n);
         }
         for (int k = 0; k < 4; ++k) fill_zero(c, w.vec[l][k], L.n);
     }
@@ -557,6 +617,9 @@ void mg_prepare(ds_context* c, const std::vector<uint8_t>& code) {
 }
 
 // z = M^-1 r, the V-cycle from level 0 (r: level 0's right-hand side), with the partials of <r, z>.
+// Level 0: pre-smoothing, restriction; each coarser level but the last: one launch down (its
+// right-hand side to the next level's); the coarsest: the dense solve; back up one launch per
+// coarser level (its result from the next level's); level 0: the two post sweeps.
 void mg_precondition(ds_context* c, const float* r, float* z) {
     MgWork& w = c->mg;
     PcgWork& pw = c->pcg;
@@ -564,31 +627,23 @@ void mg_precondition(ds_context* c, const float* r, float* z) {
     const int Lc = w.levels - 1;
     hipStream_t s = c->stream;
     const PcgStatus* st = pw.status;
-    auto rhs = [&](int l) -> const float* { return l == 0 ? r : w.vec[l][0]; };
-    for (int l = 0; l < Lc; ++l) {
-        const MgLevel& L = w.lv[l];
-        const MgLevel& C = w.lv[l + 1];
-        if (l == 0) {
-            mg_pre_kernel<true><<<xcd_blocks_for(L.n), kBlock, 0, s>>>(L, per, r, w.vec[0][1], st);
-            mg_restrict_kernel<true><<<xcd_blocks_for(C.n), kBlock, 0, s>>>(L, C, per, r, w.vec[0][1], w.vec[1][0], st);
-        } else {
-            mg_pre_kernel<false><<<xcd_blocks_for(L.n), kBlock, 0, s>>>(L, per, rhs(l), w.vec[l][1], st);
-            mg_restrict_kernel<false><<<xcd_blocks_for(C.n), kBlock, 0, s>>>(L, C, per, rhs(l), w.vec[l][1], w.vec[l + 1][0], st);
-        }
+    if (Lc == 0) {
+        mg_dense_kernel<<<w.lv[0].n, kBlock, 0, s>>>(w.lv[0].n, w.dense, r, z, st);
+        mg_dot_kernel<<<pw.blocks, kMgThreads, 0, s>>>(w.lv[0].n, r, z, pw.partials, st);
+        check_launch("multigrid direct solve");
+        return;
     }
-    mg_dense_kernel<<<w.lv[Lc].n, kBlock, 0, s>>>(w.lv[Lc].n, w.dense, rhs(Lc), Lc == 0 ? z : w.vec[Lc][3], st);
-    for (int l = Lc - 1; l >= 0; --l) {
-        const MgLevel& L = w.lv[l];
-        const MgLevel& C = w.lv[l + 1];
-        if (l == 0) {
-            mg_post1_kernel<true><<<xcd_blocks_for(L.n), kBlock, 0, s>>>(L, C, per, r, w.vec[0][1], w.vec[1][3], w.vec[0][2], st);
-            mg_post2_fine_kernel<<<pw.blocks, kMgThreads, 0, s>>>(L, per, r, w.vec[0][2], z, pw.partials, st);
-        } else {
-            mg_post1_kernel<false><<<xcd_blocks_for(L.n), kBlock, 0, s>>>(L, C, per, rhs(l), w.vec[l][1], w.vec[l + 1][3], w.vec[l][2], st);
-            mg_post2_kernel<<<xcd_blocks_for(L.n), kBlock, 0, s>>>(L, per, rhs(l), w.vec[l][2], w.vec[l][3], st);
-        }
-    }
-    if (Lc == 0) mg_dot_kernel<<<pw.blocks, kMgThreads, 0, s>>>(w.lv[0].n, r, z, pw.partials, st);
+    const MgLevel& L0 = w.lv[0];
+    mg_pre0_kernel<<<xcd_blocks_for(L0.n), kBlock, 0, s>>>(L0, per, r, w.vec[0][1], st);
+    mg_restrict0_kernel<<<xcd_blocks_for(8ull * w.lv[1].n), kBlock, 0, s>>>(L0, w.lv[1], per, r, w.vec[0][1], w.vec[1][0], st);
+    for (int l = 1; l < Lc; ++l)
+        mg_down_kernel<<<xcd_blocks_for(8ull * w.lv[l + 1].n), kBlock, 0, s>>>(w.lv[l], w.lv[l + 1], per, w.vec[l][0], w.vec[l + 1][0],
+                                                                                st);
+    mg_dense_kernel<<<w.lv[Lc].n, kBlock, 0, s>>>(w.lv[Lc].n, w.dense, w.vec[Lc][0], w.vec[Lc][3], st);
+    for (int l = Lc - 1; l >= 1; --l)
+        mg_up_kernel<<<xcd_blocks_for(w.lv[l].n), kBlock, 0, s>>>(w.lv[l], w.lv[l + 1], per, w.vec[l][0], w.vec[l + 1][3], w.vec[l][3], st);
+    mg_post0_kernel<<<xcd_blocks_for(L0.n), kBlock, 0, s>>>(L0, w.lv[1], per, r, w.vec[0][1], w.vec[1][3], w.vec[0][2], st);
+    mg_post0_last_kernel<<<pw.blocks, kMgThreads, 0, s>>>(L0, per, r, w.vec[0][2], z, pw.partials, st);
     check_launch("multigrid V-cycle");
 }
 
