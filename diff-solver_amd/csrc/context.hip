@@ -527,6 +527,7 @@ ds_status ds_context_destroy(ds_context* c) {
         set_error("ds_context_destroy: stream synchronisation failed");
         st = DS_ERROR_RUNTIME;
     }
+    mg_release(c);
     for (void* p : c->allocations) (void)hipFree(p);
     unet_release(c->unet);
     if (c->pcg.host_status) (void)hipHostFree(c->pcg.host_status);

@@ -93,6 +93,8 @@ struct PcgStatus {
     uint32_t aborted;       // 1 when a resident launch's grid barrier exceeded its bound
     double bb;              // ||b||^2
     double rr;              // last reduced ||r||^2 (recursive or true)
+    uint32_t k;             // DS_CG_MULTIGRID: the iteration whose V-cycle ran last (advanced on the device)
+    uint32_t unused;
 };
 
 // HIP-event timing of sampled PCG kernel launches (kernel 0 = direction, 1 = update, 2 = the
@@ -170,6 +172,7 @@ struct PcgWork {
     // vectors on chip (pcg_resident.hpp; DS_CG_SINGLE_REDUCTION_RESIDENT), when the grid allows it
     bool resident = false;
     uint32_t* resident_sync = nullptr;   // ResidentSync (2 words)
+    uint32_t mg_expect = 0;      // DS_CG_MULTIGRID: iterations of the last converged solve
 };
 
 // DS_CG_MULTIGRID (pcg_mg.hip): one level of the aggregation hierarchy. Level 0 is the pressure
@@ -195,6 +198,16 @@ struct MgWork {
     float* vec[kMgMaxLevels][4]{};
     double* dense = nullptr;        // inverse of the coarsest operator, n x n (symmetric)
     float* z = nullptr;             // the preconditioned residual (n + 1, zero pad)
+    // one iteration (V-cycle, direction, update) captured as a HIP graph per solution pair and
+    // tolerance: a kernel launch costs the host about as long as the kernels of a coarse level
+    // take on the device, a graph launch replays all thirteen for about the cost of one
+    struct Graph {
+        const float* xh;
+        const float* xl;
+        double tol2;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
 };
 
 }  // namespace ds
@@ -333,10 +346,17 @@ void pcg_resident_prepare(ds_context* c);
 // iteration z = M^-1 r (the V-cycle, <r, z> partials into pcg.partials), the direction kernel
 // (convergence test, p = z + beta p_old, <p, A p> partials) and the update kernel (solution pair
 // x = (xh, xl), residual, ||r||^2 partials).
+// The kernels take the iteration index from the status (PcgStatus::k, advanced by the V-cycle's
+// first kernel), so one captured iteration replays for every k: mg_set_iteration sets it (the
+// V-cycle of iteration k runs after k - 1), mg_iteration launches the captured iteration (captured
+// on first use for the pair and tolerance), mg_release destroys the captured graphs.
 void mg_prepare(ds_context* c, const std::vector<uint8_t>& code);
 void mg_precondition(ds_context* c, const float* r, float* z);
-void mg_direction(ds_context* c, const float* z, double tol2, uint32_t k, bool first, bool check_only);
-void mg_update(ds_context* c, uint32_t k, float* xh, float* xl);
+void mg_direction(ds_context* c, const float* z, double tol2, bool check_only);
+void mg_update(ds_context* c, float* xh, float* xl);
+void mg_set_iteration(ds_context* c, uint32_t k_before);
+void mg_iteration(ds_context* c, float* xh, float* xl, double tol2);
+void mg_release(ds_context* c);
 // DS_PROJECTION_RBGS (rbgs.hip): the diagonal table, then the solve of a right-hand-side kind
 // (1 forward, 2 the unrolled reverse, 3 tangent) into xh with xl = 0.
 void rbgs_prepare(ds_context* c);

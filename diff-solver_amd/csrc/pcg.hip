@@ -738,14 +738,21 @@ ds_solve_info pcg_iterate_mg(ds_context* c, float* xh, float* xl, double tol, ui
     ds_solve_info info{};
     info.warm_started = warm ? 1u : 0u;
     info.guess_rejected = rejected ? 1u : 0u;
+    // ||b||^2 (and ||r0||^2) from the set-up's partials into the status; iteration 0 next
+    pcg_finalize_kernel<<<1, kBlock, 0, c->stream>>>(w.partials, w.blocks, w.status, 1u);
+    mg_set_iteration(c, ~0u);
     uint32_t k = 0;
-    bool first = true;
     uint32_t restarts = 0;
     Profiler& prof = c->profiler;
     while (true) {
-        uint32_t batch = 1;
+        // Batches are queued one ahead of the status the host reads, and the iterations queued
+        // behind the stop cost their launches: the first batch runs to a little before the
+        // previous solve's count (a V-cycle's convergence rate hardly changes between the solves
+        // of a trajectory), the later ones two iterations at a time.
+        uint32_t batch = w.mg_expect > 4u ? w.mg_expect - 3u : 2u;
         int slot = 0, pending = -1;
         bool stop_seen = false;
+        PcgStatus st{};
         while (k < max_it && !stop_seen) {
             const uint32_t nb = std::min(batch, max_it - k);
             for (uint32_t j = 0; j < nb; ++j, ++k) {
@@ -755,10 +762,7 @@ ds_solve_info pcg_iterate_mg(ds_context* c, float* xh, float* xl, double tol, ui
                     e1 = prof.take();
                 }
                 if (e0) DS_HIP(hipEventRecord(e0, c->stream));
-                mg_precondition(c, w.r, z);
-                mg_direction(c, z, tol2, k, first, false);
-                first = false;
-                mg_update(c, k, xh, xl);
+                mg_iteration(c, xh, xl, tol2);
                 if (e1) {
                     DS_HIP(hipEventRecord(e1, c->stream));
                     prof.recs.push_back({5, e0, e1});
@@ -771,19 +775,22 @@ ds_solve_info pcg_iterate_mg(ds_context* c, float* xh, float* xl, double tol, ui
             DS_HIP(hipEventRecord(w.batch_event[slot], c->stream));
             if (pending >= 0) {
                 DS_HIP(hipEventSynchronize(w.batch_event[pending]));
-                if (w.host_status[pending].stop) stop_seen = true;
+                if (w.host_status[pending].stop) {
+                    stop_seen = true;
+                    st = w.host_status[pending];
+                }
             }
             pending = slot;
             slot ^= 1;
-            batch = std::min<uint32_t>(batch * 2u, 8u);
+            batch = 2u;
         }
-        PcgStatus st = read_status(c);
+        // the stopping launch's status when the host saw it, else the last batch's
+        if (!stop_seen) st = read_status(c);
         if (!st.stop && k >= max_it) {
             // one more stopping test on the last update's residual (its V-cycle gives the RZ
             // partials the test reads but does not use)
             mg_precondition(c, w.r, z);
-            mg_direction(c, z, tol2, k, first, true);
-            first = false;
+            mg_direction(c, z, tol2, true);
             check_launch("mg_direction_kernel");
             st = read_status(c);
         }
@@ -793,6 +800,7 @@ ds_solve_info pcg_iterate_mg(ds_context* c, float* xh, float* xl, double tol, ui
         pcg_finalize_kernel<<<1, kBlock, 0, c->stream>>>(w.partials, w.blocks, w.status, 0u);
         check_launch("pcg_true_residual_kernel");
         const PcgStatus fin = read_status(c);
+        if (st.stop) w.mg_expect = done;
         info.iterations = done;
         info.rhs_norm = std::sqrt(fin.bb);
         info.final_residual = std::sqrt(std::max(0.0, fin.rr));
@@ -804,6 +812,7 @@ ds_solve_info pcg_iterate_mg(ds_context* c, float* xh, float* xl, double tol, ui
         // the recursive residual met the tolerance but the true one (now in r) does not: continue
         // from it with the previous direction discarded
         fill_zero(c, w.p[(k + 1u) & 1u], c->geo.n);
+        mg_set_iteration(c, k - 1u);
         pcg_clear_stop_kernel<<<1, 64, 0, c->stream>>>(w.status);
         check_launch("pcg_clear_stop_kernel");
     }

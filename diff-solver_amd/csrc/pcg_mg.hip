@@ -47,6 +47,7 @@ constexpr float kOver0 = 1.8f;    // scale of the coarse correction prolonged to
 constexpr float kOmegaC = 0.9f;   // coarser levels
 constexpr float kOverC = 1.5f;
 constexpr uint32_t kMgThreads = 512;   // the kernels over contiguous ranges of level 0
+constexpr uint32_t kMgUnroll = 4;      // cells per thread whose loads are in flight together there
 
 __device__ __forceinline__ void mg_coords(const MgLevel& L, uint32_t i, int& x, int& y, int& z) {
     const uint32_t t = fdiv(i, L.dx);
@@ -56,16 +57,50 @@ __device__ __forceinline__ void mg_coords(const MgLevel& L, uint32_t i, int& x, 
     z = (int)u;
 }
 
-// 1 / d for the level-0 diagonal counts 1..6, 0 off the unknowns (a chain of selects).
-__device__ __forceinline__ float mg_inv_count(uint32_t d) {
-    float v = 0.0f;
-    v = d == 1u ? 1.0f : v;
-    v = d == 2u ? 0.5f : v;
-    v = d == 3u ? (1.0f / 3.0f) : v;
-    v = d == 4u ? 0.25f : v;
-    v = d == 5u ? 0.2f : v;
-    v = d == 6u ? (1.0f / 6.0f) : v;
-    return v;
+// omega / d for a level-0 diagonal count (the hardware reciprocal: every use of a cell's value is
+// the same, so the smoother stays symmetric). Counts of 0 (not an unknown, or the zero pad) give
+// omega; the values they scale are 0 there.
+__device__ __forceinline__ float mg_wdinv(uint32_t d) { return kOmega0 * __builtin_amdgcn_rcpf((float)max(d, 1u)); }
+
+// Level-0 neighbours of cell i = (x, y, z) in the order -x, +x, -y, +y, -z, +z. A periodic axis
+// wraps (on an extent of 1 both neighbours are i itself, as in the matrix-vector product); a
+// neighbour beyond a non-periodic axis is index n, the zero pad every level-0 vector and the code
+// carry, so it reads 0 and adds nothing: no branch per neighbour.
+struct Nb0 {
+    uint32_t j[6];
+};
+__device__ __forceinline__ Nb0 mg_nb0(const MgLevel& L, uint32_t per, int x, int y, int z, uint32_t i) {
+    const uint32_t n = L.n, sy = (uint32_t)L.nx, sz = (uint32_t)L.nx * (uint32_t)L.ny;
+    Nb0 s;
+    s.j[0] = x > 0 ? i - 1u : ((per & 1u) ? i + (uint32_t)(L.nx - 1) : n);
+    s.j[1] = x + 1 < L.nx ? i + 1u : ((per & 1u) ? i - (uint32_t)(L.nx - 1) : n);
+    s.j[2] = y > 0 ? i - sy : ((per & 2u) ? i + (uint32_t)(L.ny - 1) * sy : n);
+    s.j[3] = y + 1 < L.ny ? i + sy : ((per & 2u) ? i - (uint32_t)(L.ny - 1) * sy : n);
+    s.j[4] = z > 0 ? i - sz : ((per & 4u) ? i + (uint32_t)(L.nz - 1) * sz : n);
+    s.j[5] = z + 1 < L.nz ? i + sz : ((per & 4u) ? i - (uint32_t)(L.nz - 1) * sz : n);
+    return s;
+}
+
+// Level-1 aggregates of cell (x, y, z)'s level-0 neighbours, in mg_nb0's order (a neighbour beyond
+// a non-periodic axis gets the cell's own aggregate: its value is not used), and of the cell itself.
+__device__ __forceinline__ uint32_t mg_nb0_parents(const MgLevel& L, const MgLevel& C, uint32_t per, int x, int y, int z, uint32_t (&P)[6]) {
+    const int X = x >> 1, Y = y >> 1, Z = z >> 1;
+    const uint32_t sy = (uint32_t)C.nx, sz = (uint32_t)C.nx * (uint32_t)C.ny;
+    const uint32_t P0 = (uint32_t)X + sy * (uint32_t)Y + sz * (uint32_t)Z;
+    int q;
+    q = x > 0 ? x - 1 : ((per & 1u) ? L.nx - 1 : x);
+    P[0] = P0 + (uint32_t)((q >> 1) - X);
+    q = x + 1 < L.nx ? x + 1 : ((per & 1u) ? 0 : x);
+    P[1] = P0 + (uint32_t)((q >> 1) - X);
+    q = y > 0 ? y - 1 : ((per & 2u) ? L.ny - 1 : y);
+    P[2] = P0 + (uint32_t)((q >> 1) - Y) * sy;
+    q = y + 1 < L.ny ? y + 1 : ((per & 2u) ? 0 : y);
+    P[3] = P0 + (uint32_t)((q >> 1) - Y) * sy;
+    q = z > 0 ? z - 1 : ((per & 4u) ? L.nz - 1 : z);
+    P[4] = P0 + (uint32_t)((q >> 1) - Z) * sz;
+    q = z + 1 < L.nz ? z + 1 : ((per & 4u) ? 0 : z);
+    P[5] = P0 + (uint32_t)((q >> 1) - Z) * sz;
+    return P0;
 }
 
 // Calls f(j, q, w) for every neighbour j of cell i = (x, y, z) on level L: q its coordinates, w the
@@ -106,15 +141,6 @@ __device__ __forceinline__ float mg_offdiag(const MgLevel& L, uint32_t per, int 
     return s;
 }
 
-// The same on level 0 in FP64 (exact for FP32 values: A p with the integer stencil carries no
-// rounding into the residual recurrence).
-template <typename F>
-__device__ __forceinline__ double mg_offdiag64(const MgLevel& L, uint32_t per, int x, int y, int z, uint32_t i, F&& value) {
-    double s = 0.0;
-    mg_for_neighbours<true>(L, per, x, y, z, i, [&](uint32_t j, const int (&)[3], float) { s -= (double)value(j); });
-    return s;
-}
-
 // Index on level C of the aggregate holding the finer cell with coordinates q.
 __device__ __forceinline__ uint32_t mg_parent(const MgLevel& C, const int (&q)[3]) {
     return (uint32_t)(q[0] >> 1) + (uint32_t)C.nx * ((uint32_t)(q[1] >> 1) + (uint32_t)C.ny * (uint32_t)(q[2] >> 1));
@@ -131,9 +157,11 @@ __device__ __forceinline__ void mg_range(uint32_t n, uint32_t& lb, uint32_t& sta
 
 // Level 0, two damped Jacobi sweeps from zero for A x = r: x1 = w D^-1 r, x = x1 + w D^-1 (r - A x1),
 // with x1 at the neighbours formed on the fly.
+// Advances the iteration index (the first kernel of every V-cycle; nothing else reads it here).
 __global__ __launch_bounds__(kBlock) void mg_pre0_kernel(MgLevel L, uint32_t per, const float* __restrict__ r, float* __restrict__ x,
-                                                         const PcgStatus* status) {
+                                                         PcgStatus* status) {
     if (status->stop) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0) status->k = status->k + 1u;
     const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
     if (i >= L.n) return;
     const uint32_t d = L.code[i];
@@ -143,11 +171,14 @@ __global__ __launch_bounds__(kBlock) void mg_pre0_kernel(MgLevel L, uint32_t per
     }
     int cx, cy, cz;
     mg_coords(L, i, cx, cy, cz);
-    auto x1 = [&](uint32_t j, const int (&)[3]) { return kOmega0 * mg_inv_count(L.code[j]) * r[j]; };
-    const float ri = r[i], inv = mg_inv_count(d);
-    const float xi1 = kOmega0 * inv * ri;
-    const float ax1 = __fmaf_rn((float)d, xi1, mg_offdiag<true>(L, per, cx, cy, cz, i, x1));
-    x[i] = __fmaf_rn(kOmega0 * inv, ri - ax1, xi1);
+    const Nb0 nb = mg_nb0(L, per, cx, cy, cz, i);
+    float s1 = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) s1 += mg_wdinv(L.code[nb.j[k]]) * r[nb.j[k]];
+    const float ri = r[i], w = mg_wdinv(d);
+    const float xi1 = w * ri;
+    const float ax1 = __fmaf_rn((float)d, xi1, -s1);
+    x[i] = __fmaf_rn(w, ri - ax1, xi1);
 }
 
 // Restriction with one thread per cell: thread t handles child k = t mod 8 of aggregate I = t / 8
@@ -186,9 +217,11 @@ __global__ __launch_bounds__(kBlock) void mg_restrict0_kernel(MgLevel L, MgLevel
     if (mg_child(L, C, t, I, q, i)) {
         const uint32_t d = L.code[i];
         if (d != 0u) {
-            const float ax =
-                __fmaf_rn((float)d, x[i], mg_offdiag<true>(L, per, q[0], q[1], q[2], i, [&](uint32_t j, const int (&)[3]) { return x[j]; }));
-            v = r[i] - ax;
+            const Nb0 nb = mg_nb0(L, per, q[0], q[1], q[2], i);
+            float sx = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sx += x[nb.j[k]];
+            v = r[i] - __fmaf_rn((float)d, x[i], -sx);
         }
     }
     v = mg_group8_sum(v);
@@ -257,11 +290,18 @@ __global__ __launch_bounds__(kBlock) void mg_post0_kernel(MgLevel L, MgLevel C, 
     }
     int cx, cy, cz;
     mg_coords(L, i, cx, cy, cz);
-    auto xc = [&](uint32_t j, const int (&q)[3]) { return L.code[j] ? __fmaf_rn(kOver0, e[mg_parent(C, q)], x[j]) : 0.0f; };
-    const int c[3]{cx, cy, cz};
-    const float xi = xc(i, c);
-    const float axc = __fmaf_rn((float)d, xi, mg_offdiag<true>(L, per, cx, cy, cz, i, xc));
-    y[i] = __fmaf_rn(kOmega0 * mg_inv_count(d), r[i] - axc, xi);
+    const Nb0 nb = mg_nb0(L, per, cx, cy, cz, i);
+    uint32_t P[6];
+    const uint32_t Pi = mg_nb0_parents(L, C, per, cx, cy, cz, P);
+    float sx = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        const uint32_t j = nb.j[k];
+        sx += L.code[j] ? __fmaf_rn(kOver0, e[P[k]], x[j]) : 0.0f;
+    }
+    const float xi = __fmaf_rn(kOver0, e[Pi], x[i]);
+    const float axc = __fmaf_rn((float)d, xi, -sx);
+    y[i] = __fmaf_rn(mg_wdinv(d), r[i] - axc, xi);
 }
 
 // Level 0, second sweep z = y + w D^-1 (r - A y), over contiguous block ranges, with the partials of
@@ -274,30 +314,37 @@ __global__ __launch_bounds__(kMgThreads) void mg_post0_last_kernel(MgLevel L, ui
     uint32_t lb, start, end;
     mg_range(L.n, lb, start, end);
     double rz = 0.0;
-    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
-        const uint32_t d = L.code[i];
-        if (d == 0u) {
-            z[i] = 0.0f;
-            continue;
+    for (uint32_t base = start + threadIdx.x; base < end; base += kMgUnroll * blockDim.x) {
+#pragma unroll
+        for (int m = 0; m < kMgUnroll; ++m) {
+            const uint32_t i0 = base + (uint32_t)m * blockDim.x;
+            const bool valid = i0 < end;
+            const uint32_t i = min(i0, L.n - 1u);
+            const uint32_t d = L.code[i];
+            int cx, cy, cz;
+            mg_coords(L, i, cx, cy, cz);
+            const Nb0 nb = mg_nb0(L, per, cx, cy, cz, i);
+            float sy = 0.0f;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sy += y[nb.j[k]];
+            const float yi = y[i], ri = r[i];
+            const float zi = d ? __fmaf_rn(mg_wdinv(d), ri - __fmaf_rn((float)d, yi, -sy), yi) : 0.0f;
+            if (valid) z[i] = zi;
+            rz = valid ? __fma_rn((double)ri, (double)zi, rz) : rz;
         }
-        int cx, cy, cz;
-        mg_coords(L, i, cx, cy, cz);
-        const float yi = y[i], ri = r[i];
-        const float ay = __fmaf_rn((float)d, yi, mg_offdiag<true>(L, per, cx, cy, cz, i, [&](uint32_t j, const int (&)[3]) { return y[j]; }));
-        const float zi = __fmaf_rn(kOmega0 * mg_inv_count(d), ri - ay, yi);
-        z[i] = zi;
-        rz = __fma_rn((double)ri, (double)zi, rz);
     }
     double v[1]{rz};
     block_sum<1>(v, lds);
     if (threadIdx.x == 0) partials[kPartRZ * kPcgMaxBlocks + lb] = v[0];
 }
 
-// Coarsest level: x = A^-1 r in FP64, one block per row of the inverse.
+// Coarsest level: x = A^-1 r in FP64, one block per row of the inverse. When level 0 is the coarsest
+// this is the V-cycle's first kernel and advances the iteration index.
 __global__ __launch_bounds__(kBlock) void mg_dense_kernel(uint32_t n, const double* __restrict__ inv, const float* __restrict__ r,
-                                                          float* __restrict__ x, const PcgStatus* status) {
+                                                          float* __restrict__ x, PcgStatus* status, uint32_t advance) {
     __shared__ double lds[kMaxWaves];
     if (status->stop) return;
+    if (advance && blockIdx.x == 0 && threadIdx.x == 0) status->k = status->k + 1u;
     const uint32_t I = blockIdx.x;
     double s = 0.0;
     for (uint32_t j = threadIdx.x; j < n; j += blockDim.x) s = __fma_rn(inv[(size_t)I * n + j], (double)r[j], s);
@@ -325,51 +372,55 @@ __global__ __launch_bounds__(kMgThreads) void mg_dot_kernel(uint32_t n, const fl
 // the V-cycle), p = z + beta p_old (p = z when k = 0), partials of <p, A p>; A p is formed from p
 // at the neighbours on the fly.
 __global__ __launch_bounds__(kMgThreads) void mg_direction_kernel(MgLevel L, uint32_t per, const float* __restrict__ z,
-                                                                  const float* __restrict__ p_old, float* __restrict__ p_new,
+                                                                  float* __restrict__ p0, float* __restrict__ p1,
                                                                   double* __restrict__ partials, double* __restrict__ hist,
-                                                                  PcgStatus* status, double tol2, uint32_t k, uint32_t first,
-                                                                  uint32_t check_only) {
-    __shared__ double lds[3 * kMaxWaves];
+                                                                  PcgStatus* status, double tol2, uint32_t check_only) {
+    __shared__ double lds[2 * kMaxWaves];
     if (status->stop) return;
-    const double* parts[3]{partials + kPartBB * kPcgMaxBlocks, partials + kPartRR * kPcgMaxBlocks, partials + kPartRZ * kPcgMaxBlocks};
-    double red[3];
-    reduce_partials<3>(parts, gridDim.x, red, lds);
-    const double bb = first ? red[0] : status->bb, rho = red[1], rz = red[2];
+    const uint32_t k = status->k;
+    const double* parts[2]{partials + kPartRR * kPcgMaxBlocks, partials + kPartRZ * kPcgMaxBlocks};
+    double red[2];
+    reduce_partials<2>(parts, gridDim.x, red, lds);
+    const double bb = status->bb, rho = red[0], rz = red[1];
     uint32_t lb, start, end;
     mg_range(L.n, lb, start, end);
     const bool converged = !(rho > tol2 * bb);
     if (converged || check_only) {
         if (lb == 0 && threadIdx.x == 0) {
-            if (first) status->bb = bb;
             status->rr = rho;
             status->iterations = k;
             if (converged) status->stop = 1u;
         }
         return;
     }
-    if (lb == 0 && threadIdx.x == 0) {
-        if (first) status->bb = bb;
-        hist[2 * k] = rz;
-    }
+    if (lb == 0 && threadIdx.x == 0) hist[2 * k] = rz;
+    float* __restrict__ p_new = (k & 1u) ? p1 : p0;
+    const float* __restrict__ p_old = (k & 1u) ? p0 : p1;
     const bool with_old = k > 0;
     const float beta = with_old ? (float)(rz / hist[2 * (k - 1)]) : 0.0f;
     const uint8_t* __restrict__ code = L.code;
     // z and p_old are 0 off the unknowns, so p_new is too
     auto pn = [&](uint32_t j) -> float { return with_old ? __fmaf_rn(beta, p_old[j], z[j]) : z[j]; };
     double pq = 0.0;
-    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
-        const uint32_t d = code[i];
-        if (d == 0u) {
-            p_new[i] = 0.0f;
-            continue;
+    for (uint32_t base = start + threadIdx.x; base < end; base += kMgUnroll * blockDim.x) {
+#pragma unroll
+        for (int m = 0; m < kMgUnroll; ++m) {
+            const uint32_t i0 = base + (uint32_t)m * blockDim.x;
+            const bool valid = i0 < end;
+            const uint32_t i = min(i0, L.n - 1u);
+            const uint32_t d = code[i];
+            int cx, cy, cz;
+            mg_coords(L, i, cx, cy, cz);
+            const float pi = pn(i);   // 0 off the unknowns
+            if (valid) p_new[i] = pi;
+            // A p as the update forms it (exactly), so alpha and the residual recurrence agree
+            const Nb0 nb = mg_nb0(L, per, cx, cy, cz, i);
+            double sp = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sp += (double)pn(nb.j[k]);
+            const double q = (double)d * (double)pi - sp;
+            pq = valid ? __fma_rn((double)pi, q, pq) : pq;
         }
-        int cx, cy, cz;
-        mg_coords(L, i, cx, cy, cz);
-        const float pi = pn(i);
-        p_new[i] = pi;
-        // A p as the update forms it (exactly), so alpha and the residual recurrence agree
-        const double q = (double)d * (double)pi + mg_offdiag64(L, per, cx, cy, cz, i, pn);
-        pq = __fma_rn((double)pi, q, pq);
     }
     double v[1]{pq};
     block_sum<1>(v, lds);
@@ -381,11 +432,14 @@ __global__ __launch_bounds__(kMgThreads) void mg_direction_kernel(MgLevel L, uin
 // the recursive residual follows the true one to FP32 rounding of r itself: with the near-null
 // mode of the matrix the solution is some 10^3 ||b|| large, and an FP32 increment (or an FP32
 // A p) would leave a gap of about 1e-4 ||b|| between them after the first, largest steps.
-__global__ __launch_bounds__(kMgThreads) void mg_update_kernel(MgLevel L, uint32_t per, const float* __restrict__ p, float* __restrict__ r,
-                                                               float* __restrict__ xh, float* __restrict__ xl, double* __restrict__ partials,
-                                                               const double* __restrict__ hist, const PcgStatus* status, uint32_t k) {
+__global__ __launch_bounds__(kMgThreads) void mg_update_kernel(MgLevel L, uint32_t per, const float* __restrict__ p0,
+                                                               const float* __restrict__ p1, float* __restrict__ r, float* __restrict__ xh,
+                                                               float* __restrict__ xl, double* __restrict__ partials,
+                                                               const double* __restrict__ hist, const PcgStatus* status) {
     __shared__ double lds[kMaxWaves];
     if (status->stop) return;
+    const uint32_t k = status->k;
+    const float* __restrict__ p = (k & 1u) ? p1 : p0;
     const double* parts[1]{partials + kPartPQ * kPcgMaxBlocks};
     double red[1];
     reduce_partials<1>(parts, gridDim.x, red, lds);
@@ -393,20 +447,31 @@ __global__ __launch_bounds__(kMgThreads) void mg_update_kernel(MgLevel L, uint32
     uint32_t lb, start, end;
     mg_range(L.n, lb, start, end);
     double rr = 0.0;
-    for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
-        const uint32_t d = L.code[i];
-        if (d == 0u) continue;
-        int cx, cy, cz;
-        mg_coords(L, i, cx, cy, cz);
-        const float pi = p[i];
-        const double q = (double)d * (double)pi + mg_offdiag64(L, per, cx, cy, cz, i, [&](uint32_t j) { return p[j]; });
-        const double xv = __fma_rn(alpha, (double)pi, (double)xh[i] + (double)xl[i]);
-        const float hi = (float)xv;
-        xh[i] = hi;
-        xl[i] = (float)(xv - (double)hi);
-        const float ri = (float)__fma_rn(-alpha, q, (double)r[i]);
-        r[i] = ri;
-        rr = __fma_rn((double)ri, (double)ri, rr);
+    for (uint32_t base = start + threadIdx.x; base < end; base += kMgUnroll * blockDim.x) {
+#pragma unroll
+        for (int m = 0; m < kMgUnroll; ++m) {
+            const uint32_t i0 = base + (uint32_t)m * blockDim.x;
+            const uint32_t i = min(i0, L.n - 1u);
+            const uint32_t d = L.code[i];
+            const bool live = i0 < end && d != 0u;
+            int cx, cy, cz;
+            mg_coords(L, i, cx, cy, cz);
+            const Nb0 nb = mg_nb0(L, per, cx, cy, cz, i);
+            const float pi = p[i];
+            double sp = 0.0;
+#pragma unroll
+            for (int k = 0; k < 6; ++k) sp += (double)p[nb.j[k]];
+            const double q = (double)d * (double)pi - sp;
+            const double xv = __fma_rn(alpha, (double)pi, (double)xh[i] + (double)xl[i]);
+            const float hi = (float)xv;
+            const float ri = (float)__fma_rn(-alpha, q, (double)r[i]);
+            if (live) {
+                xh[i] = hi;
+                xl[i] = (float)(xv - (double)hi);
+                r[i] = ri;
+            }
+            rr = live ? __fma_rn((double)ri, (double)ri, rr) : rr;
+        }
     }
     double v[1]{rr};
     block_sum<1>(v, lds);
@@ -603,11 +668,12 @@ void mg_prepare(ds_context* c, const std::vector<uint8_t>& code) {
             w.vec[l][0] = device_alloc<float>(c, L.n);
             w.vec[l][3] = device_alloc<float>(c, L.n);
         } else {
-            // the pre-smoothed iterate and the iterate after the first post sweep
-            w.vec[l][1] = device_alloc<float>(c, L.n);
-            w.vec[l][2] = device_alloc<float>(c, L.n);
+            // the pre-smoothed iterate and the iterate after the first post sweep, each with the
+            // zero pad element n that out-of-domain neighbours read
+            w.vec[l][1] = device_alloc<float>(c, (size_t)L.n + 1);
+            w.vec[l][2] = device_alloc<float>(c, (size_t)L.n + 1);
         }
-        for (int k = 0; k < 4; ++k) fill_zero(c, w.vec[l][k], L.n);
+        for (int k = 0; k < 4; ++k) fill_zero(c, w.vec[l][k], (size_t)L.n + (l == 0 ? 1 : 0));
     }
     const std::vector<double> inv = dense_inverse(levels.back(), per);
     w.dense = device_alloc<double>(c, inv.size());
@@ -626,9 +692,9 @@ void mg_precondition(ds_context* c, const float* r, float* z) {
     const uint32_t per = periodic_mask(c);
     const int Lc = w.levels - 1;
     hipStream_t s = c->stream;
-    const PcgStatus* st = pw.status;
+    PcgStatus* st = pw.status;
     if (Lc == 0) {
-        mg_dense_kernel<<<w.lv[0].n, kBlock, 0, s>>>(w.lv[0].n, w.dense, r, z, st);
+        mg_dense_kernel<<<w.lv[0].n, kBlock, 0, s>>>(w.lv[0].n, w.dense, r, z, st, 1u);
         mg_dot_kernel<<<pw.blocks, kMgThreads, 0, s>>>(w.lv[0].n, r, z, pw.partials, st);
         check_launch("multigrid direct solve");
         return;
@@ -639,7 +705,7 @@ void mg_precondition(ds_context* c, const float* r, float* z) {
     for (int l = 1; l < Lc; ++l)
         mg_down_kernel<<<xcd_blocks_for(8ull * w.lv[l + 1].n), kBlock, 0, s>>>(w.lv[l], w.lv[l + 1], per, w.vec[l][0], w.vec[l + 1][0],
                                                                                 st);
-    mg_dense_kernel<<<w.lv[Lc].n, kBlock, 0, s>>>(w.lv[Lc].n, w.dense, w.vec[Lc][0], w.vec[Lc][3], st);
+    mg_dense_kernel<<<w.lv[Lc].n, kBlock, 0, s>>>(w.lv[Lc].n, w.dense, w.vec[Lc][0], w.vec[Lc][3], st, 0u);
     for (int l = Lc - 1; l >= 1; --l)
         mg_up_kernel<<<xcd_blocks_for(w.lv[l].n), kBlock, 0, s>>>(w.lv[l], w.lv[l + 1], per, w.vec[l][0], w.vec[l + 1][3], w.vec[l][3], st);
     mg_post0_kernel<<<xcd_blocks_for(L0.n), kBlock, 0, s>>>(L0, w.lv[1], per, r, w.vec[0][1], w.vec[1][3], w.vec[0][2], st);
@@ -647,17 +713,68 @@ void mg_precondition(ds_context* c, const float* r, float* z) {
     check_launch("multigrid V-cycle");
 }
 
-void mg_direction(ds_context* c, const float* z, double tol2, uint32_t k, bool first, bool check_only) {
+void mg_direction(ds_context* c, const float* z, double tol2, bool check_only) {
     PcgWork& w = c->pcg;
-    mg_direction_kernel<<<w.blocks, kMgThreads, 0, c->stream>>>(c->mg.lv[0], periodic_mask(c), z, w.p[(k + 1u) & 1u], w.p[k & 1u],
-                                                               w.partials, w.hist, w.status, tol2, k, first ? 1u : 0u,
-                                                               check_only ? 1u : 0u);
+    mg_direction_kernel<<<w.blocks, kMgThreads, 0, c->stream>>>(c->mg.lv[0], periodic_mask(c), z, w.p[0], w.p[1], w.partials, w.hist,
+                                                               w.status, tol2, check_only ? 1u : 0u);
 }
 
-void mg_update(ds_context* c, uint32_t k, float* xh, float* xl) {
+void mg_update(ds_context* c, float* xh, float* xl) {
     PcgWork& w = c->pcg;
-    mg_update_kernel<<<w.blocks, kMgThreads, 0, c->stream>>>(c->mg.lv[0], periodic_mask(c), w.p[k & 1u], w.r, xh, xl, w.partials, w.hist,
-                                                            w.status, k);
+    mg_update_kernel<<<w.blocks, kMgThreads, 0, c->stream>>>(c->mg.lv[0], periodic_mask(c), w.p[0], w.p[1], w.r, xh, xl, w.partials,
+                                                            w.hist, w.status);
+}
+
+namespace {
+
+__global__ void mg_set_iteration_kernel(PcgStatus* status, uint32_t k) {
+    if (threadIdx.x == 0) status->k = k;
+}
+
+constexpr size_t kMgMaxGraphs = 8;
+
+}  // namespace
+
+void mg_set_iteration(ds_context* c, uint32_t k_before) {
+    mg_set_iteration_kernel<<<1, 64, 0, c->stream>>>(c->pcg.status, k_before);
+    check_launch("mg_set_iteration_kernel");
+}
+
+void mg_iteration(ds_context* c, float* xh, float* xl, double tol2) {
+    MgWork& w = c->mg;
+    hipGraphExec_t exec = nullptr;
+    for (const MgWork::Graph& g : w.graphs)
+        if (g.xh == xh && g.xl == xl && g.tol2 == tol2) exec = g.exec;
+    if (!exec) {
+        if (w.graphs.size() == kMgMaxGraphs) {
+            (void)hipGraphExecDestroy(w.graphs.front().exec);
+            w.graphs.erase(w.graphs.begin());
+        }
+        hipGraph_t graph = nullptr;
+        DS_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        try {
+            mg_precondition(c, c->pcg.r, w.z);
+            mg_direction(c, w.z, tol2, false);
+            mg_update(c, xh, xl);
+            check_launch("multigrid iteration (capture)");
+        } catch (...) {
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(c->stream, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            throw;
+        }
+        DS_HIP(hipStreamEndCapture(c->stream, &graph));
+        const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        DS_HIP(e);
+        w.graphs.push_back({xh, xl, tol2, exec});
+    }
+    DS_HIP(hipGraphLaunch(exec, c->stream));
+}
+
+void mg_release(ds_context* c) {
+    for (const MgWork::Graph& g : c->mg.graphs) (void)hipGraphExecDestroy(g.exec);
+    c->mg.graphs.clear();
 }
 
 }  // namespace ds
