@@ -181,7 +181,8 @@ __global__ __launch_bounds__(kBlock) void mg_pre0_kernel(MgLevel L, uint32_t per
     x[i] = __fmaf_rn(w, ri - ax1, xi1);
 }
 
-// Restriction with one thread per cell: thread t handles child k = t mod 8 of aggregate I = t / 8
+// Restriction on the coarser levels with one thread per cell (there are few aggregates to spread
+// the work over): thread t handles child k = t mod 8 of aggregate I = t / 8
 // (k = 1 x-bit + 2 y-bit + 4 z-bit), so the 8 lanes of an aligned group share I and their values are
 // summed with three butterfly shuffles (the same association in every lane). Sets the child's cell
 // coordinates and index; false when the child lies beyond an odd extent (it contributes 0).
@@ -205,27 +206,32 @@ __device__ __forceinline__ float mg_group8_sum(float v) {
 }
 
 // Level 0 to 1: rc_I = sum of (r - A x)_i over the cells i of aggregate I (x: the pre-smoothed
-// iterate). Grid: 8 threads per aggregate.
+// iterate), one thread per aggregate (consecutive lanes read consecutive pairs of cells of each
+// row: coalesced, unlike one lane per cell of a 2 x 2 x 2 block). Children beyond an odd extent are
+// clamped onto the last cell and masked out, so every load is issued unconditionally.
 __global__ __launch_bounds__(kBlock) void mg_restrict0_kernel(MgLevel L, MgLevel C, uint32_t per, const float* __restrict__ r,
                                                               const float* __restrict__ x, float* __restrict__ rc, const PcgStatus* status) {
     if (status->stop) return;
-    const uint32_t t = xcd_block() * blockDim.x + threadIdx.x;
-    if ((t >> 3) >= C.n) return;   // whole groups of 8 leave together
-    uint32_t I, i;
-    int q[3];
-    float v = 0.0f;
-    if (mg_child(L, C, t, I, q, i)) {
-        const uint32_t d = L.code[i];
-        if (d != 0u) {
-            const Nb0 nb = mg_nb0(L, per, q[0], q[1], q[2], i);
-            float sx = 0.0f;
+    const uint32_t I = xcd_block() * blockDim.x + threadIdx.x;
+    if (I >= C.n) return;
+    int X, Y, Z;
+    mg_coords(C, I, X, Y, Z);
+    float s = 0.0f;
 #pragma unroll
-            for (int k = 0; k < 6; ++k) sx += x[nb.j[k]];
-            v = r[i] - __fmaf_rn((float)d, x[i], -sx);
-        }
+    for (int k = 0; k < 8; ++k) {
+        const int qx = 2 * X + (k & 1), qy = 2 * Y + ((k >> 1) & 1), qz = 2 * Z + (k >> 2);
+        const bool inside = qx < L.nx && qy < L.ny && qz < L.nz;
+        const int cx = min(qx, L.nx - 1), cy = min(qy, L.ny - 1), cz = min(qz, L.nz - 1);
+        const uint32_t i = (uint32_t)cx + (uint32_t)L.nx * ((uint32_t)cy + (uint32_t)L.ny * (uint32_t)cz);
+        const uint32_t d = L.code[i];
+        const Nb0 nb = mg_nb0(L, per, cx, cy, cz, i);
+        float sx = 0.0f;
+#pragma unroll
+        for (int m = 0; m < 6; ++m) sx += x[nb.j[m]];
+        const float v = r[i] - __fmaf_rn((float)d, x[i], -sx);
+        s += (inside && d != 0u) ? v : 0.0f;
     }
-    v = mg_group8_sum(v);
-    if ((t & 7u) == 0u) rc[I] = v;
+    rc[I] = s;
 }
 
 // A coarser level l >= 1 on the way down (V(1,1)): the pre-smoothing x1 = w D^-1 r is formed on the
@@ -701,7 +707,7 @@ void mg_precondition(ds_context* c, const float* r, float* z) {
     }
     const MgLevel& L0 = w.lv[0];
     mg_pre0_kernel<<<xcd_blocks_for(L0.n), kBlock, 0, s>>>(L0, per, r, w.vec[0][1], st);
-    mg_restrict0_kernel<<<xcd_blocks_for(8ull * w.lv[1].n), kBlock, 0, s>>>(L0, w.lv[1], per, r, w.vec[0][1], w.vec[1][0], st);
+    mg_restrict0_kernel<<<xcd_blocks_for(w.lv[1].n), kBlock, 0, s>>>(L0, w.lv[1], per, r, w.vec[0][1], w.vec[1][0], st);
     for (int l = 1; l < Lc; ++l)
         mg_down_kernel<<<xcd_blocks_for(8ull * w.lv[l + 1].n), kBlock, 0, s>>>(w.lv[l], w.lv[l + 1], per, w.vec[l][0], w.vec[l + 1][0],
                                                                                 st);
