@@ -198,10 +198,11 @@ def main():
                          "network modes when its weights exist)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "diff-solver_amd", "data", "warm_start_plume.npz"),
                     help="warm-start network parameters (for --warm-start cnn)")
-    ap.add_argument("--cg-variant", choices=["single", "two", "resident", "s2", "mg"], default="s2",
+    ap.add_argument("--cg-variant", choices=["single", "two", "resident", "s2", "mg"], default="mg",
                     help="PCG recurrence: single-reduction (one fused kernel per iteration), resident (the same "
                          "recurrence as one cooperative launch per replacement interval where the grid allows), "
-                         "two-reduction, or s2 (s-step: two iterations per fused launch)")
+                         "two-reduction, s2 (s-step: two iterations per fused launch), or mg (preconditioned by an "
+                         "aggregation multigrid V-cycle instead of Jacobi)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 10 device-to-device copies of MIB MiB (the reference for HBM counter scaling)")
     args = ap.parse_args()

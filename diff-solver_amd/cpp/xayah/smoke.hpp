@@ -157,7 +157,7 @@ struct Configuration {
     double cg_tolerance{1e-6};
     std::uint32_t cg_max_iterations{0u};
     WarmStart warm_start{WarmStart::none};
-    CgVariant cg_variant{CgVariant::s_step};
+    CgVariant cg_variant{CgVariant::multigrid};
 };
 
 struct ScalarField {

@@ -100,7 +100,7 @@ class Scene:
 
     def config(self, tolerance: float = 1e-6, max_iterations: int = 0, solver: int = abi.PROJECTION_PCG,
                warm_start: int = abi.WARM_START_NONE, rbgs_iterations: int = 80,
-               replacement_interval: int = 0, cg_variant: int = abi.CG_S_STEP) -> abi.Config:
+               replacement_interval: int = 0, cg_variant: int = abi.CG_MULTIGRID) -> abi.Config:
         cfg = abi.Config()
         cfg.resolution[:] = list(self.resolution)
         cfg.cell_size = self.cell_size

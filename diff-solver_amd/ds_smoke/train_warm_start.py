@@ -160,15 +160,18 @@ def train(ctx, R, params, batch=8, lr=1e-3, minutes=8.0, epochs=1000, seed=0, lo
     return out, history, step
 
 
-def evaluate(weights, grid, runs, steps, seed, device=0, log=print, solids=0, levels=0):
+def evaluate(weights, grid, runs, steps, seed, device=0, log=print, solids=0, levels=0, cg_variant=None):
     """Held-out plumes at `grid`: for each forward step's problem A x = b (b = A p, p the step's
     pressure to 1e-8), CG iterations to 1e-6 from x0 = 0, the network's guess, the extrapolation of
-    the previous solutions and the network's correction of it."""
+    the previous solutions and the network's correction of it. The iterations are counted on
+    Jacobi-preconditioned CG (the s-step form) unless cg_variant names another: the network was
+    trained to save those; the multigrid preconditioner leaves it little to save."""
     import torch
 
     from . import abi, scenes
     from .solver import Context
 
+    variant = abi.CG_S_STEP if cg_variant is None else cg_variant
     it = {"plain": [], "network": [], "extrapolate": [], "extrapolate_network": []}
     t_net = []
     for run in range(runs):
@@ -176,7 +179,7 @@ def evaluate(weights, grid, runs, steps, seed, device=0, log=print, solids=0, le
         ctx = Context(sc.config(tolerance=1e-8, warm_start=abi.WARM_START_EXTRAPOLATE), abi.MODE_FORWARD, device)
         net = Context(sc.config(tolerance=1e-6, warm_start=abi.WARM_START_CNN_ALONE), abi.MODE_FORWARD, device)
         net.warm_start_load(weights, 1.0, levels=levels)
-        plain = Context(sc.config(tolerance=1e-6), abi.MODE_FORWARD, device)
+        plain = Context(sc.config(tolerance=1e-6, cg_variant=variant), abi.MODE_FORWARD, device)
         params = ctx.allocate_parameters(sc.params)
         ctl = ctx.control_from_host(scenes.control(sc))
         state = ctx.state_from_host(scenes.initial_state(sc))

@@ -4,7 +4,9 @@ The PCG kernels tile x by 64 cells; a block rebuilds the cells beyond its tile's
 neighbours' data, so nx > 64 that is not a multiple of 64 exercises both an interior seam and a
 ragged last tile. 136 x 40 x 24 (tiles of 64, 64 and 8 cells) with spheres that straddle the
 seams at x = 64 and x = 128, closed and periodic in x (operators.cu:852-902 for the periodic
-neighbour rule): ds_pcg_solve and forward_step against the FP64 oracle.
+neighbour rule): ds_pcg_solve and forward_step against the FP64 oracle, with the default multigrid
+preconditioner (its aggregation hierarchy 136 x 40 x 24, 68 x 20 x 12, 34 x 10 x 6, 17 x 5 x 3 has
+odd extents on every axis) and with the single-reduction Jacobi form (the tiled fused kernel).
 """
 import numpy as np
 import pytest
@@ -30,18 +32,21 @@ def seam_scene(periodic_x):
     return sc
 
 
-@pytest.fixture(scope="module", params=[False, True], ids=["closed", "periodic_x"])
+@pytest.fixture(scope="module", params=[(False, "mg"), (True, "mg"), (False, "single"), (True, "single")],
+                ids=["closed-mg", "periodic_x-mg", "closed-single", "periodic_x-single"])
 def seam(request):
     import oracle as O
     from ds_smoke import abi
     from ds_smoke.solver import Context
 
-    sc = seam_scene(request.param)
+    periodic, variant = request.param
+    sc = seam_scene(periodic)
+    cg = {"mg": abi.CG_MULTIGRID, "single": abi.CG_SINGLE_REDUCTION}[variant]
     # solves to 1e-8 unless a test asks for 1e-6: the 136-cell box is ill-conditioned enough that a
     # 1e-6 residual leaves solution errors of a few 1e-5
-    ctx = Context(sc.config(tolerance=1e-8), abi.MODE_DIFFERENTIABLE)
+    ctx = Context(sc.config(tolerance=1e-8, cg_variant=cg), abi.MODE_DIFFERENTIABLE)
     orc = O.Oracle(sc, tolerance=1e-11)
-    return request.param, sc, ctx, orc
+    return periodic, sc, ctx, orc
 
 
 def test_pcg_across_tiles(seam):

@@ -68,7 +68,8 @@ def _run(mode):
     sc = scenes.plume(128)
     warm = {"plain": abi.WARM_START_NONE, "cnn_alone": abi.WARM_START_CNN_ALONE, "extrapolate": abi.WARM_START_EXTRAPOLATE,
             "cnn": abi.WARM_START_CNN}[mode]
-    ctx = Context(sc.config(tolerance=1e-6, warm_start=warm), abi.MODE_DIFFERENTIABLE)
+    # iterations of Jacobi-preconditioned CG (the s-step form), which the network was trained to save
+    ctx = Context(sc.config(tolerance=1e-6, warm_start=warm, cg_variant=abi.CG_S_STEP), abi.MODE_DIFFERENTIABLE)
     if mode.startswith("cnn"):
         _weights()
         warm_start.load_into(ctx, WEIGHTS)
