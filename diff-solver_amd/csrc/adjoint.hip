@@ -451,12 +451,22 @@ void launch_advect_scalar_vjp(ds_context* c, int nf, int first_field, const floa
 void launch_advect_velocity_vjp(ds_context* c, CF3 u, CF3 oa, F3 va) {
     const Vel3 v{{u.p[0], u.p[1], u.p[2]}};
     const Geo& g = c->geo;
+    // profiling: every sample_every-th call has each axis launch bracketed by HIP events
+    Profiler& prof = c->profiler;
+    const bool sample = prof.every && (prof.launches[6] / 3) % prof.every == 0 && prof.used + 6 <= 6000;
     auto launch = [&]<int A>() {
         const int e[3]{g.nx + (A == 0), g.ny + (A == 1), g.nz + (A == 2)};   // face extents of axis A
         const uint64_t bricks = (uint64_t)((e[0] + kVjpBX - 1) / kVjpBX) * ((e[1] + kVjpBY - 1) / kVjpBY) * ((e[2] + kVjpBZ - 1) / kVjpBZ);
         const uint32_t G = (uint32_t)((bricks + 7) & ~7ull);
+        hipEvent_t e0 = sample ? prof.take() : nullptr, e1 = sample ? prof.take() : nullptr;
+        if (e0) DS_HIP(hipEventRecord(e0, c->stream));
         if (c->has_solids) advect_velocity_vjp_kernel<A, true><<<G, kBlock, 0, c->stream>>>(g, c->d_vb, c->mask, v, oa.p[A], va);
         else advect_velocity_vjp_kernel<A, false><<<G, kBlock, 0, c->stream>>>(g, c->d_vb, c->mask, v, oa.p[A], va);
+        if (e0 && e1) {
+            DS_HIP(hipEventRecord(e1, c->stream));
+            prof.recs.push_back({6, e0, e1});
+        }
+        if (prof.every) prof.launches[6]++;
     };
     launch.template operator()<0>();
     launch.template operator()<1>();

@@ -100,8 +100,9 @@ struct PcgStatus {
 // HIP-event timing of sampled PCG kernel launches (kernel 0 = direction, 1 = update, 2 = the
 // fused single-reduction iteration).
 // 3 = the resident single-reduction launch (units: iterations), 4 = the s-step launch, 5 = one
-// multigrid-preconditioned iteration (V-cycle, direction and update)
-constexpr int kProfiledKernels = 6;
+// multigrid-preconditioned iteration (V-cycle, direction and update), 6 = one launch of the
+// velocity advection VJP (one face axis)
+constexpr int kProfiledKernels = 7;
 struct Profiler {
     uint32_t every = 0;
     uint64_t launches[kProfiledKernels]{};

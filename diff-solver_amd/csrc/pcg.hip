@@ -1069,7 +1069,7 @@ ds_status ds_profile_read(ds_context* c, ds_kernel_timing* out, uint32_t capacit
         }
         // the resident launch is reported per iteration: launches and samples count iterations
         const char* names[kProfiledKernels]{"pcg_direction_kernel", "pcg_update_kernel", "pcg_fused_kernel", "pcg_resident_kernel",
-                                            "pcg_s2_kernel", "pcg_mg_iteration"};
+                                            "pcg_s2_kernel", "pcg_mg_iteration", "advect_velocity_vjp_kernel"};
         uint32_t n = 0;
         for (int k = 0; k < kProfiledKernels; ++k) {   // kernels that ran
             if (prof.launches[k] == 0 || n >= capacity) continue;
