@@ -80,11 +80,12 @@ typedef struct ds_collider {
  * (operators.cu:885-902,1004-1013), kept as a reference mode. */
 enum { DS_PROJECTION_PCG = 0, DS_PROJECTION_RBGS = 1 };
 
-/* PCG recurrence: s-step (two iterations per fused kernel and per global reduction; what the C++
- * facade and the Python host select by default), single-reduction (Chronopoulos-Gear: one fused
- * kernel and one global reduction per iteration; the value of a zero-initialised ds_config) or the
- * two-reduction textbook form (two kernels per iteration). All iterate the same
- * Jacobi-preconditioned CG; they differ only in rounding. */
+/* PCG form: multigrid-preconditioned (what the C++ facade and the Python host select by default),
+ * or Jacobi-preconditioned in one of four recurrences: s-step (two iterations per fused kernel and
+ * per global reduction), single-reduction (Chronopoulos-Gear: one fused kernel and one global
+ * reduction per iteration; the value of a zero-initialised ds_config), its resident variant, or the
+ * two-reduction textbook form (two kernels per iteration). The Jacobi forms differ only in rounding;
+ * every form solves the same system to the same true-residual test. */
 /* DS_CG_SINGLE_REDUCTION: one FP64 reduction and one fused kernel launch per iteration.
  * DS_CG_SINGLE_REDUCTION_RESIDENT: the same recurrence as one cooperative launch per
  * residual-replacement interval with the vectors held on chip, where the grid allows it (nx a
@@ -100,7 +101,8 @@ enum { DS_PROJECTION_PCG = 0, DS_PROJECTION_RBGS = 1 };
  * aggregates, Galerkin coarse operators built at context creation, two damped Jacobi sweeps before
  * and after each coarse correction, the coarsest level of at most 512 cells solved exactly). Same
  * stopping test on the true residual; a few tens of iterations where the Jacobi forms need about a
- * thousand (128^3 plume). No counterpart in the reference, which iterates Jacobi-PCG. */
+ * thousand (128^3 plume). One iteration is captured as a HIP graph on the context's stream. No
+ * counterpart in the reference, which iterates red-black Gauss-Seidel (DS_PROJECTION_RBGS). */
 enum { DS_CG_SINGLE_REDUCTION = 0, DS_CG_TWO_REDUCTION = 1, DS_CG_SINGLE_REDUCTION_RESIDENT = 2, DS_CG_S_STEP = 3, DS_CG_MULTIGRID = 4 };
 
 /* Initial guess of the forward and adjoint pressure solves. */
