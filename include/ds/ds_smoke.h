@@ -1,6 +1,6 @@
 /*
  * ds_smoke.h — C ABI of the MI355X (gfx950) smoke solver: forward step, hand-written VJP,
- * Jacobi-PCG pressure projection and its adjoint, checkpointed trajectory gradient.
+ * preconditioned-CG pressure projection and its adjoint, checkpointed trajectory gradient.
  *
  * Every entry point takes plain pointers and sizes. Field pointers are DEVICE pointers in the
  * reference's layout (cuda_types.h:7-129, operators.cu:36-47): one allocation per component,
@@ -75,8 +75,8 @@ typedef struct ds_collider {
     float temperature;
 } ds_collider;
 
-/* Pressure solver. PCG is the build's solver (Jacobi-preconditioned CG on the reference's
- * matrix, operators.cu:852-902); RBGS is the reference's fixed-count red-black Gauss-Seidel
+/* Pressure solver. PCG is the build's solver (preconditioned CG on the reference's matrix,
+ * operators.cu:852-902; the preconditioner per cg_variant); RBGS is the reference's fixed-count red-black Gauss-Seidel
  * (operators.cu:885-902,1004-1013), kept as a reference mode. */
 enum { DS_PROJECTION_PCG = 0, DS_PROJECTION_RBGS = 1 };
 
