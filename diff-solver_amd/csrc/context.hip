@@ -316,6 +316,7 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
         g.nz = (int)cfg.resolution[2];
         g.h = cfg.cell_size;
         g.inv_h = 1.0f / cfg.cell_size;
+        g.h_pow2 = is_pow2(cfg.cell_size) ? 1u : 0u;
         g.dt = cfg.time_step;
         g.n = (uint32_t)ds_config_cell_count(&cfg);
         g.foff[0] = 0;

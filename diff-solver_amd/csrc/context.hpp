@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <exception>
 #include <new>
@@ -55,6 +56,12 @@ ds_status guard(F&& body) {
 
 inline void require(bool ok, const char* message) {
     if (!ok) throw Failure{DS_ERROR_INVALID_ARGUMENT, message};
+}
+
+// True for a positive normal power of two (then p / h == p * (1 / h) exactly).
+inline bool is_pow2(float h) {
+    int e = 0;
+    return h > 0.0f && std::frexp(h, &e) == 0.5f && h >= 1.17549435e-38f && 1.0f / h < 3.4e38f;
 }
 
 inline uint32_t blocks_for(uint64_t count, uint32_t block = kBlock) { return (uint32_t)((count + block - 1) / block); }

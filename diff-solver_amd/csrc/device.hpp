@@ -40,6 +40,7 @@ __device__ __forceinline__ uint32_t fdiv(uint32_t n, const FastDiv& f) {
 struct Geo {
     int nx, ny, nz;
     float h, inv_h, dt;
+    uint32_t h_pow2;       // 1 when h is a power of two: p / h is then exactly p * inv_h
     uint32_t n;            // cells
     uint32_t nf[3];        // faces per axis
     uint32_t foff[4];      // offsets of axis 0/1/2 in the concatenated face index space, total
@@ -161,10 +162,14 @@ struct Corner8 {   // the 2x2x2 trilinear stencil: base cell and fractional offs
     float tx, ty, tz;
 };
 
-// Grid coordinates divide by h as the reference does (operators.cu:137-139,167-172): with a cell
-// size that is not a power of two, p * (1/h) can round to the other side of a cell boundary.
+// p / h: grid coordinates divide by h as the reference does (operators.cu:137-139,167-172); with a
+// cell size that is not a power of two, p * (1/h) can round to the other side of a cell boundary.
+// For a power of two the product is the same number and skips the division sequence (the branch
+// is uniform: g is a kernel argument).
+__device__ __forceinline__ float over_h(const Geo& g, float p) { return g.h_pow2 ? p * g.inv_h : p / g.h; }
+
 __device__ __forceinline__ Corner8 corners(const Geo& g, float px, float py, float pz, float ox, float oy, float oz) {
-    const float gx = px / g.h - ox, gy = py / g.h - oy, gz = pz / g.h - oz;
+    const float gx = over_h(g, px) - ox, gy = over_h(g, py) - oy, gz = over_h(g, pz) - oz;
     Corner8 c;
     const float fx = floorf(gx), fy = floorf(gy), fz = floorf(gz);
     // A non-finite or huge position (a diverged solve upstream) must not reach the index
@@ -346,9 +351,9 @@ __device__ __forceinline__ bool solid_at(float px, float py, float pz, const uin
     if (vperiodic(b, 1)) { const float L = g.ny * g.h; py -= floorf(py / L) * L; }
     if (vperiodic(b, 2)) { const float L = g.nz * g.h; pz -= floorf(pz / L) * L; }
     // clamped as floats, so a non-finite position maps to a cell of the domain (NaN to 0)
-    const int x = (int)fminf(fmaxf(floorf(px / g.h), 0.0f), (float)(g.nx - 1));
-    const int y = (int)fminf(fmaxf(floorf(py / g.h), 0.0f), (float)(g.ny - 1));
-    const int z = (int)fminf(fmaxf(floorf(pz / g.h), 0.0f), (float)(g.nz - 1));
+    const int x = (int)fminf(fmaxf(floorf(over_h(g, px)), 0.0f), (float)(g.nx - 1));
+    const int y = (int)fminf(fmaxf(floorf(over_h(g, py)), 0.0f), (float)(g.ny - 1));
+    const int z = (int)fminf(fmaxf(floorf(over_h(g, pz)), 0.0f), (float)(g.nz - 1));
     return mask[cidx(g, x, y, z)] != 0;
 }
 

@@ -256,6 +256,7 @@ ds_status ds_example_reduce_metrics_static(const ds_config* config, const uint8_
         g.nz = (int)nz;
         g.h = config->cell_size;
         g.inv_h = 1.0f / config->cell_size;
+        g.h_pow2 = is_pow2(config->cell_size) ? 1u : 0u;
         g.dt = config->time_step;
         g.n = (uint32_t)(nx * ny * nz);
         g.cx = make_fastdiv((uint32_t)nx);
