@@ -369,9 +369,13 @@ def main():
     if not args.no_cpu_baseline:
         cb = cpu_baseline(args.grid, args.sim_steps, args.checkpoint, args.cpu_sample_steps)
         if "none" in results:
-            # the like-for-like ratio: the GPU from x0 = 0 (the oracle starts every solve from 0)
-            cb["sample"] += (f"; against the GPU from x0 = 0 ({results['none']['value']:.2f} steps/s): "
-                             f"{results['none']['value'] / cb['value']:.0f}x")
+            # the GPU from x0 = 0, as the oracle starts every solve; the oracle iterates Jacobi-PCG,
+            # the GPU's default PCG form is preconditioned by a multigrid V-cycle, so the ratio holds
+            # the algorithm's saving too: the sample names both iteration counts
+            rn = results["none"]
+            cb["sample"] += (f"; against the GPU from x0 = 0 ({rn['value']:.2f} steps/s, PCG form {args.cg_variant}, "
+                             f"{rn['per_solve']['forward']:.1f}/{rn['per_solve']['adjoint']:.1f} iterations per solve): "
+                             f"{rn['value'] / cb['value']:.0f}x")
         result["cpu_baseline"] = cb
     print(json.dumps(result))
 
