@@ -387,14 +387,15 @@ const float* unet_infer(ds_context* c, const float* b, const float* xb) {
     hipStream_t s = c->stream;
     const float* r = b;
     if (xb) {
-        base_residual_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g, per, c->code, b, xb, u->resid);
+        // one cell per thread: the stencil's dependent loads of a thread are not repeated in a loop
+        base_residual_kernel<<<blocks_for(g.n), kBlock, 0, s>>>(g, per, c->code, b, xb, u->resid);
         r = u->resid;
     }
     // the network's input: r itself, or r restricted to the network's grid
     const float* rn = r;
     const uint8_t* coden = c->code;
     if (L > 0) {
-        restrict_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g, f, d1, r, u->rc);
+        restrict_kernel<<<blocks_for(n1), kBlock, 0, s>>>(g, f, d1, r, u->rc);
         rn = u->rc;
         coden = u->ccode;
     }
@@ -438,7 +439,7 @@ const float* unet_infer(ds_context* c, const float* b, const float* xb) {
     conv.template operator()<16, 16, 16>(d1, u->f1, u->skip1, 10, u->f2);
     conv.template operator()<16, 0, 16>(d1, u->f2, nullptr, 11, u->f1);
     head_kernel<<<blocks_for(n1), kBlock, 0, s>>>((uint32_t)n1, u->f1, u->head_w, u->bias[12], coden, L > 0 ? u->dc : u->dir);
-    if (L > 0) prolong_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g, f, d1, c->code, u->dc, u->dir);
+    if (L > 0) prolong_kernel<<<blocks_for(g.n), kBlock, 0, s>>>(g, f, d1, c->code, u->dc, u->dir);
     step_partials_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g, per, c->code, r, u->dir, u->step_partials);
     step_apply_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g.n, u->step_partials, kPcgBlocks, xb, u->dir, u->guess);
     check_launch("warm-start network");
