@@ -116,9 +116,26 @@ __device__ __forceinline__ uint32_t vjp_bricks(const Geo& g, int A, int& nbx, in
     return (uint32_t)nbx * nby * ((ext(g, A, 2) + kVjpBZ - 1) / kVjpBZ);
 }
 
-template <int A, bool HasSolids>
+// The start point of a face's trace is the face itself, so the trace reverse's last three scatters
+// (the start sample's adjoint, trace_reverse) have a fixed stencil: with a power-of-two cell size
+// the component A corner is the face (weight 1) and a component B != A has the four B-faces
+// {a - 1, a} x {b, b + 1} x {c} (along A, B and the third axis; weight 1/4 each). A face whose
+// stencil needs no boundary value and no periodic mapping (start_gatherable) leaves those values in
+// `start` instead of scattering them, and start_gather_kernel adds them to each target face as a
+// gather: 9 of the about 41 LDS atomics per face, which bound the kernel.
+template <int A>
+__device__ __forceinline__ bool start_gatherable(const Geo& g, uint32_t per, int x, int y, int z) {
+    const int c[3]{x, y, z}, n[3]{g.nx, g.ny, g.nz};
+    bool ok = c[A] >= 1 && c[A] <= n[A] - 1;
+#pragma unroll
+    for (int B = 0; B < 3; ++B)
+        if (B != A && (per & (1u << B))) ok = ok && c[B] + 1 < n[B];
+    return ok;
+}
+
+template <int A, bool HasSolids, bool GatherStart>
 __global__ __launch_bounds__(256) void advect_velocity_vjp_kernel(Geo g, const VBound* __restrict__ bp, const uint8_t* __restrict__ mask,
-                                                                  Vel3 u, const float* __restrict__ oa, F3 va) {
+                                                                  Vel3 u, const float* __restrict__ oa, F3 va, F3 start, uint32_t per) {
     __shared__ float win[3 * kWin];
     const VBound& b = *bp;
     int nbx, nby;
@@ -136,17 +153,36 @@ __global__ __launch_bounds__(256) void advect_velocity_vjp_kernel(Geo g, const V
     for (int f = 0; f < kVjpBZ; ++f) {
         const int z = z0 + f;
         if (x >= ex || y >= ey || z >= ez) continue;
-        const float a = oa[fidx(g, A, x, y, z)];
-        if (a == 0.0f) continue;
-        const float sx = (x + (A == 0 ? 0.0f : 0.5f)) * g.h;
-        const float sy = (y + (A == 1 ? 0.0f : 0.5f)) * g.h;
-        const float sz = (z + (A == 2 ? 0.0f : 0.5f)) * g.h;
-        Sample mid[3];
-        const TraceOut t = trace_rk2<HasSolids, true>(sx, sy, sz, u, mask, g, b, mid);
-        const Sample s = sample_face<A, true>(u.u[A], t.px, t.py, t.pz, g, b);
-        scatter_face_to<A>(sink, t.px, t.py, t.pz, a, g, b);
-        const float v1x = -(t.dx * g.dt) * (s.gx * a), v1y = -(t.dy * g.dt) * (s.gy * a), v1z = -(t.dz * g.dt) * (s.gz * a);
-        trace_reverse(g, b, sink, sx, sy, sz, t, mid, v1x, v1y, v1z);
+        const uint32_t fi = fidx(g, A, x, y, z);
+        const float a = oa[fi];
+        const bool gather = GatherStart && start_gatherable<A>(g, per, x, y, z);
+        float st[3]{0.0f, 0.0f, 0.0f};   // the start point's adjoints, gathered later
+        if (a != 0.0f) {
+            const float sx = (x + (A == 0 ? 0.0f : 0.5f)) * g.h;
+            const float sy = (y + (A == 1 ? 0.0f : 0.5f)) * g.h;
+            const float sz = (z + (A == 2 ? 0.0f : 0.5f)) * g.h;
+            Sample mid[3];
+            const TraceOut t = trace_rk2<HasSolids, true>(sx, sy, sz, u, mask, g, b, mid);
+            const Sample s = sample_face<A, true>(u.u[A], t.px, t.py, t.pz, g, b);
+            scatter_face_to<A>(sink, t.px, t.py, t.pz, a, g, b);
+            const float v1x = -(t.dx * g.dt) * (s.gx * a), v1y = -(t.dy * g.dt) * (s.gy * a), v1z = -(t.dz * g.dt) * (s.gz * a);
+            if (gather) {
+                scatter_face_to<0>(sink, t.mx, t.my, t.mz, v1x, g, b);
+                scatter_face_to<1>(sink, t.mx, t.my, t.mz, v1y, g, b);
+                scatter_face_to<2>(sink, t.mx, t.my, t.mz, v1z, g, b);
+                const float sc = -0.5f * g.dt;
+                st[0] = sc * (mid[0].gx * v1x + mid[1].gx * v1y + mid[2].gx * v1z);
+                st[1] = sc * (mid[0].gy * v1x + mid[1].gy * v1y + mid[2].gy * v1z);
+                st[2] = sc * (mid[0].gz * v1x + mid[1].gz * v1y + mid[2].gz * v1z);
+            } else {
+                trace_reverse(g, b, sink, sx, sy, sz, t, mid, v1x, v1y, v1z);
+            }
+        }
+        if (GatherStart) {
+            start.p[0][fi] = st[0];
+            start.p[1][fi] = st[1];
+            start.p[2][fi] = st[2];
+        }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < 3 * kWin; i += blockDim.x) {
@@ -159,6 +195,37 @@ __global__ __launch_bounds__(256) void advect_velocity_vjp_kernel(Geo g, const V
         float* dst = B == 0 ? va.p[0] : B == 1 ? va.p[1] : va.p[2];
         atomicAdd(dst + fidx(g, B, wx, wy, wz), v);
     }
+}
+
+// The transpose of the start stencils of axis A's faces, for every face of component B (one thread
+// per face of the three components, concatenated): adds the gathered start adjoints.
+template <int A>
+__global__ __launch_bounds__(256) void start_gather_kernel(Geo g, uint32_t per, CF3 start, F3 va) {
+    const uint32_t f = xcd_block() * blockDim.x + threadIdx.x;
+    if (f >= g.foff[3]) return;
+    for_face(g, f, [&]<int B>(uint32_t local) {
+        int t[3];
+        decode_face_axis<B>(g, local, t[0], t[1], t[2]);
+        float s = 0.0f;
+        if (B == A) {
+            if (start_gatherable<A>(g, per, t[0], t[1], t[2])) s = start.p[A][local];
+        } else {
+            const int n[3]{g.nx, g.ny, g.nz};
+#pragma unroll
+            for (int da = 0; da < 2; ++da)
+#pragma unroll
+                for (int db = -1; db <= 0; ++db) {
+                    int q[3]{t[0], t[1], t[2]};
+                    q[A] += da;
+                    q[B] += db;
+                    // an axis-A face: its own axis has n + 1 faces, the others n
+                    if (q[A] < 0 || q[A] > n[A] || q[B] < 0 || q[B] >= n[B]) continue;
+                    if (!start_gatherable<A>(g, per, q[0], q[1], q[2])) continue;
+                    s += 0.25f * start.p[B][fidx(g, A, q[0], q[1], q[2])];
+                }
+        }
+        if (s != 0.0f) va.p[B][local] += s;
+    });
 }
 
 // constrained_boundary, operators.cu:452-458
@@ -451,6 +518,12 @@ void launch_advect_scalar_vjp(ds_context* c, int nf, int first_field, const floa
 void launch_advect_velocity_vjp(ds_context* c, CF3 u, CF3 oa, F3 va) {
     const Vel3 v{{u.p[0], u.p[1], u.p[2]}};
     const Geo& g = c->geo;
+    uint32_t per = 0;   // bit d: axis d of the velocity boundary is periodic (vperiodic)
+    for (int d = 0; d < 3; ++d)
+        if (c->vb.modes[2 * d] == 3u && c->vb.modes[2 * d + 1] == 3u) per |= 1u << d;
+    // the start stencils are gathered when the context holds the buffers and the fractions are exact
+    const bool gather = c->vjp_start[0] != nullptr && g.h_pow2;
+    const F3 st{{c->vjp_start[0], c->vjp_start[1], c->vjp_start[2]}};
     // profiling: every sample_every-th call has each axis launch bracketed by HIP events
     Profiler& prof = c->profiler;
     const bool sample = prof.every && (prof.launches[6] / 3) % prof.every == 0 && prof.used + 6 <= 6000;
@@ -460,8 +533,18 @@ void launch_advect_velocity_vjp(ds_context* c, CF3 u, CF3 oa, F3 va) {
         const uint32_t G = (uint32_t)((bricks + 7) & ~7ull);
         hipEvent_t e0 = sample ? prof.take() : nullptr, e1 = sample ? prof.take() : nullptr;
         if (e0) DS_HIP(hipEventRecord(e0, c->stream));
-        if (c->has_solids) advect_velocity_vjp_kernel<A, true><<<G, kBlock, 0, c->stream>>>(g, c->d_vb, c->mask, v, oa.p[A], va);
-        else advect_velocity_vjp_kernel<A, false><<<G, kBlock, 0, c->stream>>>(g, c->d_vb, c->mask, v, oa.p[A], va);
+        if (gather) {
+            if (c->has_solids)
+                advect_velocity_vjp_kernel<A, true, true><<<G, kBlock, 0, c->stream>>>(g, c->d_vb, c->mask, v, oa.p[A], va, st, per);
+            else
+                advect_velocity_vjp_kernel<A, false, true><<<G, kBlock, 0, c->stream>>>(g, c->d_vb, c->mask, v, oa.p[A], va, st, per);
+            start_gather_kernel<A><<<xcd_blocks_for(g.foff[3]), kBlock, 0, c->stream>>>(g, per, CF3{{st.p[0], st.p[1], st.p[2]}}, va);
+        } else {
+            if (c->has_solids)
+                advect_velocity_vjp_kernel<A, true, false><<<G, kBlock, 0, c->stream>>>(g, c->d_vb, c->mask, v, oa.p[A], va, st, per);
+            else
+                advect_velocity_vjp_kernel<A, false, false><<<G, kBlock, 0, c->stream>>>(g, c->d_vb, c->mask, v, oa.p[A], va, st, per);
+        }
         if (e0 && e1) {
             DS_HIP(hipEventRecord(e1, c->stream));
             prof.recs.push_back({6, e0, e1});

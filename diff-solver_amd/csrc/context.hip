@@ -482,6 +482,10 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
                 c->forced_adj[a] = device_alloc<float>(c, g.nf[a]);
                 c->raw_adj[a] = device_alloc<float>(c, g.nf[a]);
             }
+            {
+                const uint32_t nmax = std::max({g.nf[0], g.nf[1], g.nf[2]});
+                for (int a = 0; a < 3; ++a) c->vjp_start[a] = device_alloc<float>(c, nmax);
+            }
             c->lambda_hi = device_alloc<float>(c, n);
             c->lambda_lo = device_alloc<float>(c, n);
             c->param_partials = device_alloc<double>(c, 4 * (size_t)kPcgBlocks);
