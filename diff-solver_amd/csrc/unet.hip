@@ -136,21 +136,33 @@ __global__ __launch_bounds__(256) void head_kernel(uint32_t n, const uint16_t* _
 }
 
 // Per-block partials of the step along d: <d, r> and <d, A d> (A d gathered on the fly; d = 0 off
-// the unknowns): partials[k * G + block], k = 0, 1.
+// the unknowns): partials[k * G + block], k = 0, 1. A thread takes kStepCells cells a block apart
+// (coalesced), their codes loaded first, so a wave keeps several cells' loads in flight.
+constexpr int kStepCells = 4;
 __global__ __launch_bounds__(256) void step_partials_kernel(Geo geo, uint32_t per, const uint8_t* __restrict__ code,
                                                            const float* __restrict__ r, const float* __restrict__ d,
                                                            double* __restrict__ partials) {
     __shared__ double lds[2 * kMaxWaves];
     double v[2]{0.0, 0.0};
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < geo.n; i += gridDim.x * blockDim.x) {
-        const uint32_t cd = code[i];
-        if (!cd) continue;
+    const uint32_t i0 = blockIdx.x * (blockDim.x * kStepCells) + threadIdx.x;
+    uint32_t cd[kStepCells];
+#pragma unroll
+    for (int k = 0; k < kStepCells; ++k) {
+        const uint32_t i = i0 + k * blockDim.x;
+        cd[k] = i < geo.n ? code[i] : 0u;
+    }
+#pragma unroll
+    for (int k = 0; k < kStepCells; ++k) {
+        if (!cd[k]) continue;
+        const uint32_t i = i0 + k * blockDim.x;
         int x, y, z;
         decode_cell(geo, i, x, y, z);
         const int c[3]{x, y, z}, ext[3]{geo.nx, geo.ny, geo.nz};
         const double di = d[i];
-        double s = (double)cd * di;
+        double s = (double)cd[k] * di;
+#pragma unroll
         for (int a = 0; a < 3; ++a)
+#pragma unroll
             for (int dir = -1; dir <= 1; dir += 2) {
                 int q[3]{c[0], c[1], c[2]};
                 q[a] += dir;
@@ -217,6 +229,7 @@ struct UNet {
     float* dc = nullptr;                // network output on the network grid (levels > 0)
     double* count_partials = nullptr;   // kPcgBlocks + 1
     bf16x4_t* wpk[kUNetLayerCount]{};
+    uint4* wpk32[kUNetLayerCount]{};     // 3x3x3 convolutions: the 16x16x32 packing (pack_weights32)
     float* bias[kUNetLayerCount]{};
     float* head_w = nullptr;
     // activations (fine: N voxels, mid: N/8, coarse: N/64)
@@ -227,7 +240,8 @@ struct UNet {
     float* dir = nullptr;                // network output d
     float* guess = nullptr;              // x0
     double* norm_partials = nullptr;     // kPcgBlocks
-    double* step_partials = nullptr;     // 2 * kPcgBlocks
+    double* step_partials = nullptr;     // 2 * step_blocks
+    uint32_t step_blocks = 0;            // blocks of step_partials_kernel
 };
 
 void unet_release(UNet* u) {
@@ -275,7 +289,8 @@ void unet_prepare(ds_context* c, UNet* u) {
     u->dir = unet_alloc<float>(u, nf, true);
     u->guess = unet_alloc<float>(u, nf, true);
     u->norm_partials = unet_alloc<double>(u, (size_t)kPcgBlocks, true);
-    u->step_partials = unet_alloc<double>(u, 2 * (size_t)kPcgBlocks, true);
+    u->step_blocks = (uint32_t)((nf + (size_t)kBlock * kStepCells - 1) / ((size_t)kBlock * kStepCells));
+    u->step_partials = unet_alloc<double>(u, 2 * (size_t)u->step_blocks, true);
     if (L > 0) {
         u->ccode = unet_alloc<uint8_t>(u, n, true);
         u->rc = unet_alloc<float>(u, n, true);
@@ -309,6 +324,23 @@ std::vector<uint16_t> pack_weights(const float* w, int cin, int cout, int taps, 
                         const float v = transposed ? w[((size_t)ci * cout + co) * taps + tap] : w[((size_t)co * cin + ci) * taps + tap];
                         pk[((((size_t)tap * nch + ch) * nb + b) * 64 + l) * 4 + j] = to_bf16(v);
                     }
+    return pk;
+}
+
+// 3x3x3 conv weights -> the order of conv3_tile_kernel's steps: pk[((s * NB + nb) * 64 + l) * 8 + j] =
+// W[co = 16 nb + l%16][ci][tap] with kk = 32 s + 8 (l/16) + j = tap * cs + ci (cs: the input
+// channels as stored, cin padded to 4 for the input layer), 0 for ci >= cin and tap >= 27.
+std::vector<uint16_t> pack_weights32(const float* w, int cin, int cs, int cout) {
+    const int steps = (27 * cs + 31) / 32, nb = cout / 16;
+    std::vector<uint16_t> pk((size_t)steps * nb * 64 * 8, 0);
+    for (int st = 0; st < steps; ++st)
+        for (int b = 0; b < nb; ++b)
+            for (int l = 0; l < 64; ++l)
+                for (int j = 0; j < 8; ++j) {
+                    const int kk = 32 * st + 8 * (l >> 4) + j, tap = kk / cs, ci = kk % cs, co = 16 * b + (l & 15);
+                    if (tap >= 27 || ci >= cin) continue;
+                    pk[(((size_t)st * nb + b) * 64 + l) * 8 + j] = to_bf16(w[((size_t)co * cin + ci) * 27 + tap]);
+                }
     return pk;
 }
 
@@ -351,6 +383,8 @@ void unet_load(ds_context* c, const float* params, uint64_t count, double output
     UNet* u = c->unet ? c->unet : new UNet();
     c->unet = u;
     const float* p = params;
+    std::vector<std::vector<uint16_t>> staging;   // packed weights, alive until the copies are done
+    staging.reserve(2 * kUNetLayerCount);
     for (int k = 0; k < kUNetLayerCount; ++k) {
         const UNetLayer& L = kUNetLayers[k];
         const int taps = L.kind == 0 ? 27 : L.kind == 1 ? 8 : 1;
@@ -360,9 +394,14 @@ void unet_load(ds_context* c, const float* params, uint64_t count, double output
             if (!u->head_w) u->head_w = unet_alloc<float>(u, L.cin);
             DS_HIP(hipMemcpyAsync(u->head_w, p, nw * sizeof(float), hipMemcpyHostToDevice, c->stream));
         } else {
-            const std::vector<uint16_t> pk = pack_weights(p, L.cin, L.cout, taps, L.kind == 1);
+            const std::vector<uint16_t>& pk = staging.emplace_back(pack_weights(p, L.cin, L.cout, taps, L.kind == 1));
             if (!u->wpk[k]) u->wpk[k] = unet_alloc<bf16x4_t>(u, pk.size() / 4);
             DS_HIP(hipMemcpyAsync(u->wpk[k], pk.data(), pk.size() * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream));
+            if (L.kind == 0) {
+                const std::vector<uint16_t>& pk32 = staging.emplace_back(pack_weights32(p, L.cin, L.cin < 4 ? 4 : L.cin, L.cout));
+                if (!u->wpk32[k]) u->wpk32[k] = unet_alloc<uint4>(u, pk32.size() / 8);
+                DS_HIP(hipMemcpyAsync(u->wpk32[k], pk32.data(), pk32.size() * sizeof(uint16_t), hipMemcpyHostToDevice, c->stream));
+            }
         }
         DS_HIP(hipMemcpyAsync(u->bias[k], p + nw, L.cout * sizeof(float), hipMemcpyHostToDevice, c->stream));
         p += nw + L.cout;
@@ -408,11 +447,20 @@ const float* unet_infer(ds_context* c, const float* b, const float* xb) {
         o.bias = u->bias[k];
         return o;
     };
-    // a 3x3x3 convolution with ReLU: 4 rows and every channel tile per wave where that gives the
-    // grid at least kConvWaves waves, else 1 row per wave, else 1 row and 1 channel tile per wave
-    // (the 32^3 and 16^3 levels of a 64^3 network)
+    // a 3x3x3 convolution with ReLU: the LDS-tiled kernel with bricks of 16 x 8 x 2 voxels (16 output
+    // channels), 16 x 4 x 2 (32) or 16 x 2 x 1 (64) where the grid divides into them; else the
+    // direct-load kernel with 4 rows and every channel tile per wave where that gives the grid at
+    // least kConvWaves waves, else 1 row per wave, else 1 row and 1 channel tile per wave
     constexpr uint64_t kConvWaves = 2048;
     auto conv = [&]<int C1, int C2, int COUT>(const Dims& d, const uint16_t* in1, const uint16_t* in2, int k, uint16_t* out) {
+        constexpr int TY = COUT == 16 ? 8 : COUT == 32 ? 4 : 2, TZ = COUT == 64 ? 1 : 2, NBW = COUT == 64 ? 4 : COUT / 16;
+        constexpr int WK = COUT == 64 ? 4 : COUT == 32 ? 2 : 1, RING = COUT == 16 ? 1 : 4;
+        if (d.nx % 16 == 0 && d.ny % TY == 0 && d.nz % TZ == 0) {
+            const uint64_t bricks = (uint64_t)(d.nx / 16) * (d.ny / TY) * (d.nz / TZ);
+            conv3_tile_kernel<C1, C2, COUT, TY, TZ, NBW, WK, RING, kEpiBiasRelu><<<(uint32_t)((bricks + 7) & ~7ull), 256, 0, s>>>(
+                d, 1, in1, in2, u->wpk32[k], fwd(k, out));
+            return;
+        }
         const uint64_t units = (uint64_t)(d.nx / 16) * d.ny * d.nz;   // 16-voxel x rows
         if (d.ny % 4 == 0 && units / 4 >= kConvWaves)
             conv3_kernel<C1, C2, COUT, 4, kEpiBiasRelu><<<wave_blocks(units / 4), 256, 0, s>>>(d, 1, in1, in2, u->wpk[k], fwd(k, out));
@@ -440,8 +488,8 @@ const float* unet_infer(ds_context* c, const float* b, const float* xb) {
     conv.template operator()<16, 0, 16>(d1, u->f2, nullptr, 11, u->f1);
     head_kernel<<<blocks_for(n1), kBlock, 0, s>>>((uint32_t)n1, u->f1, u->head_w, u->bias[12], coden, L > 0 ? u->dc : u->dir);
     if (L > 0) prolong_kernel<<<blocks_for(g.n), kBlock, 0, s>>>(g, f, d1, c->code, u->dc, u->dir);
-    step_partials_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g, per, c->code, r, u->dir, u->step_partials);
-    step_apply_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g.n, u->step_partials, kPcgBlocks, xb, u->dir, u->guess);
+    step_partials_kernel<<<u->step_blocks, kBlock, 0, s>>>(g, per, c->code, r, u->dir, u->step_partials);
+    step_apply_kernel<<<kPcgBlocks, kBlock, 0, s>>>(g.n, u->step_partials, u->step_blocks, xb, u->dir, u->guess);
     check_launch("warm-start network");
     return u->guess;
 }

@@ -28,6 +28,7 @@
 #include "device.hpp"
 
 typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
 typedef float floatx4_t __attribute__((ext_vector_type(4)));
 
 namespace ds {
@@ -62,6 +63,11 @@ __device__ __forceinline__ uint32_t vox(const Dims& d, int x, int y, int zg) {
 __device__ __forceinline__ floatx4_t mfma(bf16x4_t a, bf16x4_t b, floatx4_t c) {
     return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
+// v_mfma_f32_16x16x32_bf16 (gfx950): 8 BF16 per lane and operand, lane l holds A[m = l%16][k = 8(l/16)
+// + j] and B[k = 8(l/16) + j][n = l%16] (j = 0..7); D as the 16x16x16 form.
+__device__ __forceinline__ floatx4_t mfma32(uint4 a, uint4 b, floatx4_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8_t, a), __builtin_bit_cast(bf16x8_t, b), c, 0, 0, 0);
+}
 
 // Output stage of a 3x3x3 convolution kernel.
 enum ConvEpilogue : int {
@@ -78,6 +84,25 @@ struct ConvOut {
     const uint16_t* mask = nullptr;   // kEpiMask: BF16 tensor with the output's channels
     float* out_f32 = nullptr;         // kEpiSplit: channels CU .. COUT-1
 };
+
+// Writes one MFMA D fragment of a convolution: acc[i] is output channel co of voxel (x + i, y, zg).
+template <int EPI, int COUT, int CU>
+__device__ __forceinline__ void conv_store(const ConvOut& o, const Dims& d, const floatx4_t& acc, int x, int y, int zg, int co) {
+    const float bv = (EPI == kEpiBias || EPI == kEpiBiasRelu) ? o.bias[co] : 0.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const size_t v = vox(d, x + i, y, zg);
+        float val = acc[i] + bv;
+        if (EPI == kEpiBiasRelu) val = fmaxf(val, 0.0f);
+        if (EPI == kEpiMask) val = bf16_positive(o.mask[v * COUT + co]) ? val : 0.0f;
+        if (EPI == kEpiSplit) {
+            if (co < CU) o.out[v * CU + co] = to_bf16(val);
+            else o.out_f32[v * (COUT - CU) + (co - CU)] = val;
+        } else {
+            o.out[v * COUT + co] = to_bf16(val);
+        }
+    }
+}
 
 // 3x3x3 convolution, stride 1, zero padding, over the channel concatenation [in1 (C1), in2 (C2)],
 // with the weights in fragment order wpk[((tap * NCH + ch) * NB + nb) * 64 + lane]. One wave
@@ -138,23 +163,185 @@ __global__ __launch_bounds__(256) void conv3_kernel(Dims d, int batch, const uin
 #pragma unroll
     for (int t = 0; t < MT; ++t)
 #pragma unroll
-        for (int nb = 0; nb < NBW; ++nb) {
-            const int co = (nb0 + nb) * 16 + m;
-            const float bv = (EPI == kEpiBias || EPI == kEpiBiasRelu) ? o.bias[co] : 0.0f;
+        for (int nb = 0; nb < NBW; ++nb) conv_store<EPI, COUT, CU>(o, d, acc[t][nb], x0 + 4 * kq, y0 + t, zg, (nb0 + nb) * 16 + m);
+}
+
+// 3x3x3 convolution as conv3_kernel, with the block's input tile staged in LDS and the products on
+// v_mfma_f32_16x16x32_bf16. A block computes a 16 x TY x TZ brick of output voxels (rows of 16 in x)
+// over all COUT channels: it loads the brick plus a one-voxel halo, every input channel, into LDS
+// once (zeros outside the sample: the padding), and the 27 taps read it from there. K runs over
+// (tap, input channel), channel fastest, kk = tap * CS + ci with CS the stored input channels (the
+// input layer's in1 holds 4, the concatenation [in1, in2] C1 + C2), 32 per MFMA step; the taps
+// 27..31 of a last partial step read zeros. The 4 waves split the work into WK interleaved parts
+// of the steps x NG groups of NBW channel tiles x RG groups of RW rows; with WK > 1 the partial
+// sums meet in LDS and the first part's waves add them in a fixed order. Each wave keeps the
+// weights of RING steps in flight (RING = 1: the next step's, loaded before a step's products)
+// and, with RING > 1, reads the next step's LDS operands during a step's products. wpk: weights in
+// the order of the steps, wpk[(step * NB + nb) * 64 + lane] = 8 BF16 (pack_weights32). The grid is
+// the brick count rounded up to a multiple of 8 (XCD remap).
+template <int C1, int C2, int COUT, int TY, int TZ, int NBW, int WK, int RING, int EPI, int CU = 0>
+__global__ __launch_bounds__(256) void conv3_tile_kernel(Dims d, int batch, const uint16_t* __restrict__ in1,
+                                                         const uint16_t* __restrict__ in2, const uint4* __restrict__ wpk, ConvOut o) {
+    static_assert(C1 >= 4 || C2 == 0, "an input layer (fewer than 4 channels) has no second input");
+    constexpr int S1 = C1 < 4 ? 4 : C1;          // channel stride of in1
+    constexpr int CS = S1 + C2;                  // channels per voxel in K
+    static_assert(CS == 4 || (S1 % 8 == 0 && C2 % 8 == 0), "channel groups of 8");
+    constexpr int VS = CS == 4 ? 4 : CS + 8;     // LDS voxel stride (BF16): 16 lanes' reads hit distinct banks
+    constexpr int NB = COUT / 16, NG = NB / NBW, R = TY * TZ;
+    static_assert(NB % NBW == 0 && 4 % (WK * NG) == 0, "wave split");
+    constexpr int RG = 4 / (WK * NG);
+    static_assert(R % RG == 0, "row split");
+    constexpr int RW = R / RG;
+    constexpr int BX = 18, BY = TY + 2, BZ = TZ + 2, BV = BX * BY * BZ;
+    constexpr int STEPS = (27 * CS + 31) / 32;
+    constexpr int kRing = RING;
+    static_assert(RING >= 1, "weights in flight");
+    // the tile, then (WK > 1) the partial sums of the parts after the first, in the same memory
+    constexpr int TILE_BYTES = BV * VS * 2, RED_BYTES = (WK - 1) * NG * RG * RW * NBW * 64 * 16;
+    __shared__ __align__(16) uint8_t smem[TILE_BYTES > RED_BYTES ? TILE_BYTES : RED_BYTES];
+    uint16_t* tile = reinterpret_cast<uint16_t*>(smem);
+
+    const int tiles_x = d.nx / 16, tiles_y = d.ny / TY, tiles_z = d.nz / TZ;
+    uint32_t blk = xcd_block();
+    if (blk >= (uint32_t)tiles_x * tiles_y * tiles_z * batch) return;   // padding blocks (whole block)
+    const int x0 = (int)(blk % (uint32_t)tiles_x) * 16;
+    blk /= (uint32_t)tiles_x;
+    const int y0 = (int)(blk % (uint32_t)tiles_y) * TY;
+    blk /= (uint32_t)tiles_y;
+    const int z0 = (int)(blk % (uint32_t)tiles_z) * TZ;
+    const int zs = (int)(blk / (uint32_t)tiles_z) * d.nz;   // first plane of the sample
+
+    // stage the brick and its halo
+    if constexpr (CS == 4) {
+        for (int u = threadIdx.x; u < BV; u += 256) {
+            const int bx = u % BX, by = (u / BX) % BY, bz = u / (BX * BY);
+            const int x = x0 - 1 + bx, y = y0 - 1 + by, z = z0 - 1 + bz;
+            uint2 v = make_uint2(0u, 0u);
+            if (x >= 0 && x < d.nx && y >= 0 && y < d.ny && z >= 0 && z < d.nz)
+                v = *reinterpret_cast<const uint2*>(in1 + (size_t)vox(d, x, y, zs + z) * 4);
+            *reinterpret_cast<uint2*>(tile + u * VS) = v;
+        }
+    } else {
+        constexpr int U = CS / 8;   // 16-byte units per voxel
+        for (int u = threadIdx.x; u < BV * U; u += 256) {
+            const int bv = u / U, c = (u % U) * 8;
+            const int bx = bv % BX, by = (bv / BX) % BY, bz = bv / (BX * BY);
+            const int x = x0 - 1 + bx, y = y0 - 1 + by, z = z0 - 1 + bz;
+            uint4 v = make_uint4(0u, 0u, 0u, 0u);
+            if (x >= 0 && x < d.nx && y >= 0 && y < d.ny && z >= 0 && z < d.nz) {
+                const size_t vx = vox(d, x, y, zs + z);
+                v = c < S1 ? *reinterpret_cast<const uint4*>(in1 + vx * S1 + c) : *reinterpret_cast<const uint4*>(in2 + vx * C2 + (c - S1));
+            }
+            *reinterpret_cast<uint4*>(tile + bv * VS + c) = v;
+        }
+    }
+    __syncthreads();
+
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m = lane & 15, g = lane >> 4;
+    const int ks = w % WK, grp = w / WK;          // part of the steps; row x channel-tile group
+    const int nb0 = (grp % NG) * NBW, r0 = (grp / NG) * RW;
+    int rowoff[RW];   // LDS voxel of (x = m - 1, the row's y - 1, z - 1): tap (0, 0, 0) of the row
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const size_t v = vox(d, x0 + 4 * kq + i, y0 + t, zg);
-                float val = acc[t][nb][i] + bv;
-                if (EPI == kEpiBiasRelu) val = fmaxf(val, 0.0f);
-                if (EPI == kEpiMask) val = bf16_positive(o.mask[v * COUT + co]) ? val : 0.0f;
-                if (EPI == kEpiSplit) {
-                    if (co < CU) o.out[v * CU + co] = to_bf16(val);
-                    else o.out_f32[v * (COUT - CU) + (co - CU)] = val;
-                } else {
-                    o.out[v * COUT + co] = to_bf16(val);
+    for (int i = 0; i < RW; ++i) rowoff[i] = m + BX * (((r0 + i) % TY) + BY * ((r0 + i) / TY));
+    floatx4_t acc[RW][NBW];
+#pragma unroll
+    for (int i = 0; i < RW; ++i)
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb) acc[i][nb] = floatx4_t{0.0f, 0.0f, 0.0f, 0.0f};
+    auto tap_off = [](int tap) { return tap % 3 + BX * ((tap / 3) % 3 + BY * (tap / 9)); };
+    // the A operands of step s (zeros for the taps past 26)
+    auto load_a = [&](int s, uint4 (&a)[RW]) {
+        const int kk = 32 * s + 8 * g;
+        if constexpr (CS == 4) {
+            const int t0 = kk >> 2, o0 = tap_off(t0), o1 = tap_off(t0 + 1);
+#pragma unroll
+            for (int i = 0; i < RW; ++i) {
+                uint2 lo = make_uint2(0u, 0u), hi = make_uint2(0u, 0u);
+                if (t0 < 27) lo = *reinterpret_cast<const uint2*>(tile + (rowoff[i] + o0) * VS);
+                if (t0 + 1 < 27) hi = *reinterpret_cast<const uint2*>(tile + (rowoff[i] + o1) * VS);
+                a[i] = make_uint4(lo.x, lo.y, hi.x, hi.y);
+            }
+        } else {
+            const int tap = kk / CS, c0 = kk % CS, o0 = tap_off(tap);
+#pragma unroll
+            for (int i = 0; i < RW; ++i)
+                a[i] = tap < 27 ? *reinterpret_cast<const uint4*>(tile + (rowoff[i] + o0) * VS + c0) : make_uint4(0u, 0u, 0u, 0u);
+        }
+    };
+    // this wave's steps: ks, ks + WK, ...; n = their count
+    constexpr int NMAX = (STEPS + WK - 1) / WK;
+    const int n = (STEPS - ks + WK - 1) / WK;
+    uint4 ring[kRing][NBW];
+#pragma unroll
+    for (int p = 0; p < kRing; ++p)
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb)
+            ring[p][nb] = p < n ? wpk[((size_t)(ks + WK * p) * NB + nb0 + nb) * 64 + lane] : make_uint4(0u, 0u, 0u, 0u);
+    uint4 a[RW];
+    if constexpr (RING == 1) {
+        // fully unrolled: the compiler overlaps a step's LDS reads with the previous step's products
+#pragma unroll
+        for (int t = 0; t < NMAX; ++t) {
+            if (t < n) {
+                uint4 bn[NBW];
+#pragma unroll
+                for (int nb = 0; nb < NBW; ++nb)
+                    bn[nb] = t + 1 < n ? wpk[((size_t)(ks + WK * (t + 1)) * NB + nb0 + nb) * 64 + lane] : ring[0][nb];
+                load_a(ks + WK * t, a);
+#pragma unroll
+                for (int i = 0; i < RW; ++i)
+#pragma unroll
+                    for (int nb = 0; nb < NBW; ++nb) acc[i][nb] = mfma32(a[i], ring[0][nb], acc[i][nb]);
+#pragma unroll
+                for (int nb = 0; nb < NBW; ++nb) ring[0][nb] = bn[nb];
+            }
+        }
+    } else {
+        load_a(ks, a);
+#pragma unroll 1
+        for (int t0 = 0; t0 < NMAX; t0 += kRing) {
+#pragma unroll
+            for (int p = 0; p < kRing; ++p) {
+                const int t = t0 + p;
+                if (t < n) {
+                    uint4 an[RW];
+                    if (t + 1 < n) load_a(ks + WK * (t + 1), an);
+#pragma unroll
+                    for (int i = 0; i < RW; ++i)
+#pragma unroll
+                        for (int nb = 0; nb < NBW; ++nb) acc[i][nb] = mfma32(a[i], ring[p][nb], acc[i][nb]);
+                    if (t + kRing < n)
+#pragma unroll
+                        for (int nb = 0; nb < NBW; ++nb) ring[p][nb] = wpk[((size_t)(ks + WK * (t + kRing)) * NB + nb0 + nb) * 64 + lane];
+#pragma unroll
+                    for (int i = 0; i < RW; ++i) a[i] = an[i];
                 }
             }
         }
+    }
+    if constexpr (WK > 1) {
+        floatx4_t* red = reinterpret_cast<floatx4_t*>(smem);   // [(ks - 1)][grp][i][nb][lane]
+        constexpr int GR = NG * RG;
+        __syncthreads();   // every wave is done with the tile
+        if (ks > 0)
+#pragma unroll
+            for (int i = 0; i < RW; ++i)
+#pragma unroll
+                for (int nb = 0; nb < NBW; ++nb) red[((((ks - 1) * GR + grp) * RW + i) * NBW + nb) * 64 + lane] = acc[i][nb];
+        __syncthreads();
+        if (ks > 0) return;
+        for (int k = 1; k < WK; ++k)
+#pragma unroll
+            for (int i = 0; i < RW; ++i)
+#pragma unroll
+                for (int nb = 0; nb < NBW; ++nb) acc[i][nb] += red[((((k - 1) * GR + grp) * RW + i) * NBW + nb) * 64 + lane];
+    }
+#pragma unroll
+    for (int i = 0; i < RW; ++i)
+#pragma unroll
+        for (int nb = 0; nb < NBW; ++nb)
+            conv_store<EPI, COUT, CU>(o, d, acc[i][nb], x0 + 4 * g, y0 + (r0 + i) % TY, zs + z0 + (r0 + i) / TY, (nb0 + nb) * 16 + m);
 }
 
 // 2x2x2 stride-2 transposed convolution from the coarse grid dc to the fine grid (2 dc): fine voxel
