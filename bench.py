@@ -178,6 +178,19 @@ def unet_timing(ctx, sc, levels, repeats=20):
             "flops_per_inference": flops, "ms_per_inference": round(ms, 4), "dtype": "bf16 (fp32 accumulation)"}
 
 
+def network_only(grid, levels, weights):
+    """The warm-start network alone: its guess on the grid^3 plume's pressure system (random right-hand
+    side on the unknowns), the network run on the grid coarsened `levels` times."""
+    from ds_smoke import abi, scenes, warm_start
+    from ds_smoke.solver import Context
+
+    sc = scenes.plume(grid)
+    ctx = Context(sc.config(tolerance=1e-6, warm_start=abi.WARM_START_CNN, cg_variant=abi.CG_MULTIGRID), abi.MODE_DIFFERENTIABLE)
+    flat, scale, _ = warm_start.load(weights)
+    ctx.warm_start_load(flat, scale, levels=levels)
+    print(json.dumps({"workload": f"plume{grid}_network_guess", "levels": levels, **unet_timing(ctx, sc, levels)}), flush=True)
+
+
 def traffic_from_profiles(kernel):
     """HBM bytes per launch from the committed rocprofv3 counter summary, when present."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_plume128_traffic.json")
@@ -212,9 +225,15 @@ def main():
                          "aggregation multigrid V-cycle instead of Jacobi)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 10 device-to-device copies of MIB MiB (the reference for HBM counter scaling)")
+    ap.add_argument("--network-only", type=int, default=-1, metavar="LEVELS",
+                    help="only time the warm-start network's guess on the --grid plume, the grid coarsened LEVELS "
+                         "times (0, 1 or 2), and print its MFMA roofline as one JSON line")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.network_only >= 0:
+        network_only(args.grid, args.network_only, args.weights)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the hot path is single-GPU here); got --gpus {args.gpus}", file=sys.stderr)
