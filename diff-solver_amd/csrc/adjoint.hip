@@ -166,16 +166,20 @@ __global__ __launch_bounds__(256) void advect_velocity_vjp_kernel(Geo g, const V
             const Sample s = sample_face<A, true>(u.u[A], t.px, t.py, t.pz, g, b);
             scatter_face_to<A>(sink, t.px, t.py, t.pz, a, g, b);
             const float v1x = -(t.dx * g.dt) * (s.gx * a), v1y = -(t.dy * g.dt) * (s.gy * a), v1z = -(t.dz * g.dt) * (s.gz * a);
-            if (gather) {
-                scatter_face_to<0>(sink, t.mx, t.my, t.mz, v1x, g, b);
-                scatter_face_to<1>(sink, t.mx, t.my, t.mz, v1y, g, b);
-                scatter_face_to<2>(sink, t.mx, t.my, t.mz, v1z, g, b);
-                const float sc = -0.5f * g.dt;
-                st[0] = sc * (mid[0].gx * v1x + mid[1].gx * v1y + mid[2].gx * v1z);
-                st[1] = sc * (mid[0].gy * v1x + mid[1].gy * v1y + mid[2].gy * v1z);
-                st[2] = sc * (mid[0].gz * v1x + mid[1].gz * v1y + mid[2].gz * v1z);
-            } else {
-                trace_reverse(g, b, sink, sx, sy, sz, t, mid, v1x, v1y, v1z);
+            // trace_reverse, with the start point's three adjoints either kept for the gather or
+            // scattered here
+            scatter_face_to<0>(sink, t.mx, t.my, t.mz, v1x, g, b);
+            scatter_face_to<1>(sink, t.mx, t.my, t.mz, v1y, g, b);
+            scatter_face_to<2>(sink, t.mx, t.my, t.mz, v1z, g, b);
+            const float sc = -0.5f * g.dt;
+            st[0] = sc * (mid[0].gx * v1x + mid[1].gx * v1y + mid[2].gx * v1z);
+            st[1] = sc * (mid[0].gy * v1x + mid[1].gy * v1y + mid[2].gy * v1z);
+            st[2] = sc * (mid[0].gz * v1x + mid[1].gz * v1y + mid[2].gz * v1z);
+            if (!gather) {
+                scatter_face_to<0>(sink, sx, sy, sz, st[0], g, b);
+                scatter_face_to<1>(sink, sx, sy, sz, st[1], g, b);
+                scatter_face_to<2>(sink, sx, sy, sz, st[2], g, b);
+                st[0] = st[1] = st[2] = 0.0f;
             }
         }
         if (GatherStart) {
