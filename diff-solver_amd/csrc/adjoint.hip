@@ -49,39 +49,92 @@ __device__ __forceinline__ void trace_reverse(const Geo& g, const VBound& b, con
 }
 
 // advect_scalar_vjp_kernel (operators.cu:537-561) for NF fields that share one trace: the
-// velocity adjoint of both fields is summed before it is scattered (the map is linear).
+// velocity adjoint of both fields is summed before it is scattered (the map is linear). A block
+// owns a kSvBX x kSvBY x kSvBZ brick of cells and privatises its scatters as the velocity reverse
+// does: the NF fields' and the three velocity components' adds that land in a window around the
+// brick (halo kSvHalo) are LDS atomics, the rest (long traces, periodic wraps) global atomics; the
+// windows are flushed once.
+constexpr int kSvBX = 32, kSvBY = 8, kSvBZ = 2, kSvHalo = 1;
+constexpr int kSwX = kSvBX + 2 * kSvHalo + 1, kSwY = kSvBY + 2 * kSvHalo + 1, kSwZ = kSvBZ + 2 * kSvHalo + 1;
+constexpr int kSw = kSwX * kSwY * kSwZ;   // per target array: velocity components 0-2, fields 3-4
+
+struct ScalarBrickSink {
+    float* win;        // LDS, 5 * kSw
+    int ox, oy, oz;    // coordinates of window element (0, 0, 0)
+    F3 va;
+    float* s0;
+    float* s1;
+    template <int W>
+    __device__ __forceinline__ void add(int x, int y, int z, uint32_t i, float value) const {
+        const unsigned lx = (unsigned)(x - ox), ly = (unsigned)(y - oy), lz = (unsigned)(z - oz);
+        if (lx < (unsigned)kSwX && ly < (unsigned)kSwY && lz < (unsigned)kSwZ)
+            __hip_atomic_fetch_add(win + W * kSw + ((int)lz * kSwY + (int)ly) * kSwX + (int)lx, value, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+        else
+            atomicAdd((W < 3 ? va.p[W < 3 ? W : 0] : W == 3 ? s0 : s1) + i, value);
+    }
+};
+
 template <int NF, bool HasSolids>
-__global__ __launch_bounds__(256) void advect_scalar_vjp_kernel(Geo g, const VBound* __restrict__ vbp, const SBound* __restrict__ sb0p,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void advect_scalar_vjp_kernel(Geo g, const VBound* __restrict__ vbp, const SBound* __restrict__ sb0p,
                                                                 const SBound* __restrict__ sb1p, const uint8_t* __restrict__ mask,
                                                                 const float* __restrict__ src0, const float* __restrict__ src1, Vel3 u,
                                                                 const float* __restrict__ oa0, const float* __restrict__ oa1,
                                                                 float* src_adj0, float* src_adj1, F3 va) {
+    static_assert(kSvBX * kSvBY == 256, "one plane of the brick per pass of the block");
+    __shared__ float win[5 * kSw];
     const VBound& vb = *vbp;
     const SBound& sb0 = *sb0p;
     const SBound& sb1 = *sb1p;
-    const uint32_t i = xcd_block() * blockDim.x + threadIdx.x;
-    if (i >= g.n) return;
-    if (HasSolids && mask[i] != 0) return;
-    const float a0 = oa0[i];
-    const float a1 = NF == 2 ? oa1[i] : 0.0f;
-    if (a0 == 0.0f && a1 == 0.0f) return;
-    int x, y, z;
-    decode_cell(g, i, x, y, z);
-    const float sx = (x + 0.5f) * g.h, sy = (y + 0.5f) * g.h, sz = (z + 0.5f) * g.h;
-    Sample mid[3];
-    const TraceOut t = trace_rk2<HasSolids, true>(sx, sy, sz, u, mask, g, vb, mid);
-    const Sample s0 = sample_scalar<true>(src0, t.px, t.py, t.pz, g, sb0);
-    scatter_scalar(src_adj0, t.px, t.py, t.pz, a0, g, sb0);
-    float gx = s0.gx * a0, gy = s0.gy * a0, gz = s0.gz * a0;
-    if (NF == 2) {
-        const Sample s1 = sample_scalar<true>(src1, t.px, t.py, t.pz, g, sb1);
-        scatter_scalar(src_adj1, t.px, t.py, t.pz, a1, g, sb1);
-        gx += s1.gx * a1;
-        gy += s1.gy * a1;
-        gz += s1.gz * a1;
+    const uint32_t nbx = (uint32_t)(g.nx + kSvBX - 1) / kSvBX, nby = (uint32_t)(g.ny + kSvBY - 1) / kSvBY;
+    const uint32_t bricks = nbx * nby * (uint32_t)((g.nz + kSvBZ - 1) / kSvBZ);
+    const uint32_t lb = xcd_block();
+    if (lb >= bricks) return;   // padding blocks of the XCD-rounded grid (whole block)
+    const int x0 = (int)(lb % nbx) * kSvBX, y0 = (int)((lb / nbx) % nby) * kSvBY, z0 = (int)(lb / (nbx * nby)) * kSvBZ;
+    for (int k = threadIdx.x; k < 5 * kSw; k += blockDim.x) win[k] = 0.0f;
+    __syncthreads();
+    const ScalarBrickSink sink{win, x0 - kSvHalo, y0 - kSvHalo, z0 - kSvHalo, va, src_adj0, NF == 2 ? src_adj1 : src_adj0};
+    const int x = x0 + (int)(threadIdx.x % kSvBX), y = y0 + (int)(threadIdx.x / kSvBX);
+    for (int f = 0; f < kSvBZ; ++f) {
+        const int z = z0 + f;
+        if (x >= g.nx || y >= g.ny || z >= g.nz) continue;
+        const uint32_t i = cidx(g, x, y, z);
+        if (HasSolids && mask[i] != 0) continue;
+        const float a0 = oa0[i];
+        const float a1 = NF == 2 ? oa1[i] : 0.0f;
+        if (a0 == 0.0f && a1 == 0.0f) continue;
+        const float sx = (x + 0.5f) * g.h, sy = (y + 0.5f) * g.h, sz = (z + 0.5f) * g.h;
+        Sample mid[3];
+        const TraceOut t = trace_rk2<HasSolids, true>(sx, sy, sz, u, mask, g, vb, mid);
+        const Sample s0 = sample_scalar<true>(src0, t.px, t.py, t.pz, g, sb0);
+        scatter_scalar_to<3>(sink, t.px, t.py, t.pz, a0, g, sb0);
+        float gx = s0.gx * a0, gy = s0.gy * a0, gz = s0.gz * a0;
+        if (NF == 2) {
+            const Sample s1 = sample_scalar<true>(src1, t.px, t.py, t.pz, g, sb1);
+            scatter_scalar_to<4>(sink, t.px, t.py, t.pz, a1, g, sb1);
+            gx += s1.gx * a1;
+            gy += s1.gy * a1;
+            gz += s1.gz * a1;
+        }
+        const float v1x = -(t.dx * g.dt) * gx, v1y = -(t.dy * g.dt) * gy, v1z = -(t.dz * g.dt) * gz;
+        trace_reverse(g, vb, sink, sx, sy, sz, t, mid, v1x, v1y, v1z);
     }
-    const float v1x = -(t.dx * g.dt) * gx, v1y = -(t.dy * g.dt) * gy, v1z = -(t.dz * g.dt) * gz;
-    trace_reverse(g, vb, VelSink{va}, sx, sy, sz, t, mid, v1x, v1y, v1z);
+    __syncthreads();
+    for (int k = threadIdx.x; k < 5 * kSw; k += blockDim.x) {
+        const float v = win[k];
+        if (v == 0.0f) continue;
+        const int W = k / kSw, r = k - W * kSw;
+        const int wx = x0 - kSvHalo + r % kSwX, wy = y0 - kSvHalo + (r / kSwX) % kSwY, wz = z0 - kSvHalo + r / (kSwX * kSwY);
+        // only in-range targets receive window adds; the test keeps the flush in bounds regardless
+        if (W < 3) {
+            if (wx < 0 || wy < 0 || wz < 0 || wx >= ext(g, W, 0) || wy >= ext(g, W, 1) || wz >= ext(g, W, 2)) continue;
+            float* dst = W == 0 ? va.p[0] : W == 1 ? va.p[1] : va.p[2];
+            atomicAdd(dst + fidx(g, W, wx, wy, wz), v);
+        } else {
+            if (wx < 0 || wy < 0 || wz < 0 || wx >= g.nx || wy >= g.ny || wz >= g.nz) continue;
+            atomicAdd((W == 3 ? src_adj0 : src_adj1) + cidx(g, wx, wy, wz), v);
+        }
+    }
 }
 
 // advect_velocity_vjp_kernel (operators.cu:415-439) for the faces of axis A, with its scatters
@@ -499,7 +552,9 @@ __global__ __launch_bounds__(256) void integrate_force_vjp_kernel(Geo g, const u
 void launch_advect_scalar_vjp(ds_context* c, int nf, int first_field, const float* src0, const float* src1, CF3 u, const float* oa0,
                               const float* oa1, float* sa0, float* sa1, F3 va) {
     const Vel3 v{{u.p[0], u.p[1], u.p[2]}};
-    const uint32_t G = xcd_blocks_for(c->geo.n);
+    const Geo& g = c->geo;
+    const uint64_t bricks = (uint64_t)((g.nx + kSvBX - 1) / kSvBX) * ((g.ny + kSvBY - 1) / kSvBY) * ((g.nz + kSvBZ - 1) / kSvBZ);
+    const uint32_t G = (uint32_t)((bricks + 7) & ~7ull);
     if (nf == 2) {
         if (c->has_solids)
             advect_scalar_vjp_kernel<2, true><<<G, kBlock, 0, c->stream>>>(c->geo, c->d_vb, c->d_db, c->d_tb, c->mask, src0, src1, v, oa0, oa1, sa0,

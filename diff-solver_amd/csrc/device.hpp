@@ -323,6 +323,29 @@ __device__ __forceinline__ void scatter_face(float* v, float px, float py, float
     scatter_face_to<A>(GlobalSink{v}, px, py, pz, adj, g, b);
 }
 
+// scatter_scalar with a sink: the same weights and skips, the adds go to sink.add<W>(x, y, z, i, value)
+// with (x, y, z) the mapped cell and i its index (W names the sink's target array).
+template <int W, typename Sink>
+__device__ __forceinline__ void scatter_scalar_to(const Sink& sink, float px, float py, float pz, float adj, const Geo& g,
+                                                  const SBound& b) {
+    if (adj == 0.0f) return;
+    const Corner8 c = corners(g, px, py, pz, 0.5f, 0.5f, 0.5f);
+    const float w[2][3] = {{1.0f - c.tx, 1.0f - c.ty, 1.0f - c.tz}, {c.tx, c.ty, c.tz}};
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int dx = k & 1, dy = (k >> 1) & 1, dz = k >> 2;
+        int x = c.x0 + dx, y = c.y0 + dy, z = c.z0 + dz;
+        float cv;
+        if (scalar_const(g, x, y, z, b, cv)) continue;
+        const float wk = w[dx][0] * w[dy][1] * w[dz][2];
+        if (wk == 0.0f) continue;
+        x = mapc(x, g.nx, speriodic(b, 0), g.nx);
+        y = mapc(y, g.ny, speriodic(b, 1), g.ny);
+        z = mapc(z, g.nz, speriodic(b, 2), g.nz);
+        sink.template add<W>(x, y, z, cidx(g, x, y, z), adj * wk);
+    }
+}
+
 // scatter_scalar (operators.cu:196-215).
 __device__ __forceinline__ void scatter_scalar(float* v, float px, float py, float pz, float adj, const Geo& g,
                                                const SBound& b) {
