@@ -97,6 +97,7 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_setup_kernel(Geo g, uint32_t 
     uint32_t lb, start, end;
     block_range(g.n, lb, start, end);
     double bb = 0.0, rr = 0.0, rz = 0.0;
+#pragma unroll 2
     for (uint32_t i = start + threadIdx.x; i < end; i += blockDim.x) {
         const uint32_t d = code[i];
         int x, y, z;
@@ -107,8 +108,10 @@ __global__ __launch_bounds__(kPcgThreads) void pcg_setup_kernel(Geo g, uint32_t 
         float ri = bi;
         if (x0h != nullptr) {
             if (d != 0u) {
+                // loaded unconditionally (j is a cell of the grid), selected by the code
                 auto xv = [&](uint32_t j) -> double {
-                    return code[j] ? (double)x0h[j] + (x0l ? (double)x0l[j] : 0.0) : 0.0;
+                    const double v = (double)x0h[j] + (x0l ? (double)x0l[j] : 0.0);
+                    return code[j] ? v : 0.0;
                 };
                 const double ax = (double)d * xv(i) - neighbour_sum<double>(g, per, x, y, z, i, xv);
                 ri = (float)((double)bi - ax);

@@ -36,13 +36,22 @@ __device__ __forceinline__ float cell_rhs(const Geo& g, const RhsIn& in, const u
         return b;
     } else {
         auto ua = [&](int a, uint32_t f) { return in.u[a][f] + (in.extra[a] ? in.extra[a][f] : 0.0f); };
+        // every face value and neighbour flag is loaded unconditionally (the six faces of a cell
+        // exist; a neighbour outside the domain reads the cell itself and is masked off), so the
+        // loads issue together; an excluded face adds zero
+        const bool xp = x + 1 < g.nx, xm = x > 0, yp = y + 1 < g.ny, ym = y > 0, zp = z + 1 < g.nz, zm = z > 0;
+        const uint8_t mxp = mask[xp ? i + 1u : i], mxm = mask[xm ? i - 1u : i];
+        const uint8_t myp = mask[yp ? i + (uint32_t)g.nx : i], mym = mask[ym ? i - (uint32_t)g.nx : i];
+        const uint8_t mzp = mask[zp ? i + sz : i], mzm = mask[zm ? i - sz : i];
+        const float uxp = ua(0, f0 + 1u), uxm = ua(0, f0), uyp = ua(1, f1 + (uint32_t)g.nx), uym = ua(1, f1);
+        const float uzp = ua(2, f2 + sz), uzm = ua(2, f2);
         float s = 0.0f;
-        if (x + 1 < g.nx && mask[i + 1u] == 0) s += ua(0, f0 + 1u);
-        if (x > 0 && mask[i - 1u] == 0) s -= ua(0, f0);
-        if (y + 1 < g.ny && mask[i + (uint32_t)g.nx] == 0) s += ua(1, f1 + (uint32_t)g.nx);
-        if (y > 0 && mask[i - (uint32_t)g.nx] == 0) s -= ua(1, f1);
-        if (z + 1 < g.nz && mask[i + sz] == 0) s += ua(2, f2 + sz);
-        if (z > 0 && mask[i - sz] == 0) s -= ua(2, f2);
+        s += xp && mxp == 0 ? uxp : 0.0f;
+        s -= xm && mxm == 0 ? uxm : 0.0f;
+        s += yp && myp == 0 ? uyp : 0.0f;
+        s -= ym && mym == 0 ? uym : 0.0f;
+        s += zp && mzp == 0 ? uzp : 0.0f;
+        s -= zm && mzm == 0 ? uzm : 0.0f;
         return g.dt * s * g.inv_h;
     }
 }
