@@ -174,8 +174,8 @@ __device__ __forceinline__ uint32_t vjp_bricks(const Geo& g, int A, int& nbx, in
 // the component A corner is the face (weight 1) and a component B != A has the four B-faces
 // {a - 1, a} x {b, b + 1} x {c} (along A, B and the third axis; weight 1/4 each). A face whose
 // stencil needs no boundary value and no periodic mapping (start_gatherable) leaves those values in
-// `start` instead of scattering them, and start_gather_kernel adds them to each target face as a
-// gather: 9 of the about 41 LDS atomics per face, which bound the kernel.
+// `start` instead of scattering them, and start_gather_kernel (after the three axes) adds them to
+// each target face as a gather: 9 of the about 41 LDS atomics per face, which bound the kernel.
 template <int A>
 __device__ __forceinline__ bool start_gatherable(const Geo& g, uint32_t per, int x, int y, int z) {
     const int c[3]{x, y, z}, n[3]{g.nx, g.ny, g.nz};
@@ -254,33 +254,41 @@ __global__ __launch_bounds__(256) void advect_velocity_vjp_kernel(Geo g, const V
     }
 }
 
-// The transpose of the start stencils of axis A's faces, for every face of component B (one thread
-// per face of the three components, concatenated): adds the gathered start adjoints.
-template <int A>
-__global__ __launch_bounds__(256) void start_gather_kernel(Geo g, uint32_t per, CF3 start, F3 va) {
+// The transpose of the start stencils of the three axes' faces, for every face of component B (one
+// thread per face of the three components, concatenated), after the three axes' reverse kernels:
+// adds the gathered start adjoints. start[A][B]: component B's start adjoints of axis A's faces.
+struct StartAdj {
+    const float* p[3][3];
+};
+__global__ __launch_bounds__(256) void start_gather_kernel(Geo g, uint32_t per, StartAdj start, F3 va) {
     const uint32_t f = xcd_block() * blockDim.x + threadIdx.x;
     if (f >= g.foff[3]) return;
     for_face(g, f, [&]<int B>(uint32_t local) {
         int t[3];
         decode_face_axis<B>(g, local, t[0], t[1], t[2]);
+        const int n[3]{g.nx, g.ny, g.nz};
         float s = 0.0f;
-        if (B == A) {
-            if (start_gatherable<A>(g, per, t[0], t[1], t[2])) s = start.p[A][local];
-        } else {
-            const int n[3]{g.nx, g.ny, g.nz};
+        auto from_axis = [&]<int A>() {
+            if (A == B) {
+                if (start_gatherable<A>(g, per, t[0], t[1], t[2])) s += start.p[A][A][local];
+            } else {
 #pragma unroll
-            for (int da = 0; da < 2; ++da)
+                for (int da = 0; da < 2; ++da)
 #pragma unroll
-                for (int db = -1; db <= 0; ++db) {
-                    int q[3]{t[0], t[1], t[2]};
-                    q[A] += da;
-                    q[B] += db;
-                    // an axis-A face: its own axis has n + 1 faces, the others n
-                    if (q[A] < 0 || q[A] > n[A] || q[B] < 0 || q[B] >= n[B]) continue;
-                    if (!start_gatherable<A>(g, per, q[0], q[1], q[2])) continue;
-                    s += 0.25f * start.p[B][fidx(g, A, q[0], q[1], q[2])];
-                }
-        }
+                    for (int db = -1; db <= 0; ++db) {
+                        int q[3]{t[0], t[1], t[2]};
+                        q[A] += da;
+                        q[B] += db;
+                        // an axis-A face: its own axis has n + 1 faces, the others n
+                        if (q[A] < 0 || q[A] > n[A] || q[B] < 0 || q[B] >= n[B]) continue;
+                        if (!start_gatherable<A>(g, per, q[0], q[1], q[2])) continue;
+                        s += 0.25f * start.p[A][B][fidx(g, A, q[0], q[1], q[2])];
+                    }
+            }
+        };
+        from_axis.template operator()<0>();
+        from_axis.template operator()<1>();
+        from_axis.template operator()<2>();
         if (s != 0.0f) va.p[B][local] += s;
     });
 }
@@ -581,8 +589,7 @@ void launch_advect_velocity_vjp(ds_context* c, CF3 u, CF3 oa, F3 va) {
     for (int d = 0; d < 3; ++d)
         if (c->vb.modes[2 * d] == 3u && c->vb.modes[2 * d + 1] == 3u) per |= 1u << d;
     // the start stencils are gathered when the context holds the buffers and the fractions are exact
-    const bool gather = c->vjp_start[0] != nullptr && g.h_pow2;
-    const F3 st{{c->vjp_start[0], c->vjp_start[1], c->vjp_start[2]}};
+    const bool gather = c->vjp_start[0][0] != nullptr && g.h_pow2;
     // profiling: every sample_every-th call has each axis launch bracketed by HIP events
     Profiler& prof = c->profiler;
     const bool sample = prof.every && (prof.launches[6] / 3) % prof.every == 0 && prof.used + 6 <= 6000;
@@ -592,12 +599,12 @@ void launch_advect_velocity_vjp(ds_context* c, CF3 u, CF3 oa, F3 va) {
         const uint32_t G = (uint32_t)((bricks + 7) & ~7ull);
         hipEvent_t e0 = sample ? prof.take() : nullptr, e1 = sample ? prof.take() : nullptr;
         if (e0) DS_HIP(hipEventRecord(e0, c->stream));
+        const F3 st{{c->vjp_start[A][0], c->vjp_start[A][1], c->vjp_start[A][2]}};
         if (gather) {
             if (c->has_solids)
                 advect_velocity_vjp_kernel<A, true, true><<<G, kBlock, 0, c->stream>>>(g, c->d_vb, c->mask, v, oa.p[A], va, st, per);
             else
                 advect_velocity_vjp_kernel<A, false, true><<<G, kBlock, 0, c->stream>>>(g, c->d_vb, c->mask, v, oa.p[A], va, st, per);
-            start_gather_kernel<A><<<xcd_blocks_for(g.foff[3]), kBlock, 0, c->stream>>>(g, per, CF3{{st.p[0], st.p[1], st.p[2]}}, va);
         } else {
             if (c->has_solids)
                 advect_velocity_vjp_kernel<A, true, false><<<G, kBlock, 0, c->stream>>>(g, c->d_vb, c->mask, v, oa.p[A], va, st, per);
@@ -613,6 +620,12 @@ void launch_advect_velocity_vjp(ds_context* c, CF3 u, CF3 oa, F3 va) {
     launch.template operator()<0>();
     launch.template operator()<1>();
     launch.template operator()<2>();
+    if (gather) {
+        StartAdj sa;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) sa.p[a][b] = c->vjp_start[a][b];
+        start_gather_kernel<<<xcd_blocks_for(g.foff[3]), kBlock, 0, c->stream>>>(g, per, sa, va);
+    }
     check_launch("advect_velocity_vjp_kernel");
 }
 

@@ -484,7 +484,8 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
             }
             {
                 const uint32_t nmax = std::max({g.nf[0], g.nf[1], g.nf[2]});
-                for (int a = 0; a < 3; ++a) c->vjp_start[a] = device_alloc<float>(c, nmax);
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b) c->vjp_start[a][b] = device_alloc<float>(c, nmax);
             }
             c->lambda_hi = device_alloc<float>(c, n);
             c->lambda_lo = device_alloc<float>(c, n);

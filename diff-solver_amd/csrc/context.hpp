@@ -273,8 +273,8 @@ struct ds_context {
     float* forced_adj[3]{};           // adjoint of the forced velocity u*
     float* raw_adj[3]{};              // adjoint of the raw advected velocity
     // velocity advection reverse (differentiable mode, power-of-two cell size): the start point's
-    // velocity adjoint per face of the axis being reversed, one array per velocity component
-    float* vjp_start[3]{};
+    // velocity adjoint per face of axis A (first index), one array per velocity component
+    float* vjp_start[3][3]{};
     float* lambda_hi = nullptr;       // adjoint pressure solve solution (hi + lo)
     float* lambda_lo = nullptr;
     bool lambda_valid = false;
