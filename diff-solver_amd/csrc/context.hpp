@@ -25,12 +25,16 @@ struct Failure {
     std::string message;
 };
 
+// A failed runtime call becomes a Failure; the runtime's last-error slot, which the failed call
+// also set, is cleared so that the next check_launch does not report it again as a launch error.
 #define DS_HIP(call)                                                                                  \
     do {                                                                                              \
         const hipError_t ds_hip_status_ = (call);                                                     \
-        if (ds_hip_status_ != hipSuccess)                                                             \
+        if (ds_hip_status_ != hipSuccess) {                                                           \
+            (void)hipGetLastError();                                                                  \
             throw ::ds::Failure{ds_hip_status_ == hipErrorOutOfMemory ? DS_ERROR_OUT_OF_MEMORY : DS_ERROR_RUNTIME, \
                                 std::string(#call) + ": " + hipGetErrorString(ds_hip_status_)};       \
+        }                                                                                             \
     } while (0)
 
 // check_launch (operators.cu:28-30): a failed launch becomes DS_ERROR_LAUNCH with the kernel name.
