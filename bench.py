@@ -86,13 +86,14 @@ def host_threads():
         return os.cpu_count() or 1
 
 
-def cpu_baseline(grid, sim_steps, k, sample_steps):
-    """FP64 oracle (same discretisation and stopping rule) on a bounded sample of the workload."""
+def cpu_baseline(grid, sim_steps, k, sample_steps, threads=0):
+    """FP64 oracle (same discretisation and stopping rule) on a bounded sample of the workload, on
+    `threads` host threads (0: all the process may use)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from ds_smoke import scenes
 
-    threads = host_threads()
+    threads = threads or host_threads()
     O.set_threads(threads)
     sc = scenes.plume(grid)
     orc = O.Oracle(sc, tolerance=1e-6)
@@ -225,6 +226,9 @@ def main():
                          "aggregation multigrid V-cycle instead of Jacobi)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run 10 device-to-device copies of MIB MiB (the reference for HBM counter scaling)")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="host threads of the CPU baseline (0: all)")
+    ap.add_argument("--cpu-only", action="store_true",
+                    help="only time the CPU baseline (the FP64 oracle on --cpu-sample-steps steps) and print it as one JSON line")
     ap.add_argument("--network-only", type=int, default=-1, metavar="LEVELS",
                     help="only time the warm-start network's guess on the --grid plume, the grid coarsened LEVELS "
                          "times (0, 1 or 2), and print its MFMA roofline as one JSON line")
@@ -234,6 +238,10 @@ def main():
         return
     if args.network_only >= 0:
         network_only(args.grid, args.network_only, args.weights)
+        return
+    if args.cpu_only:
+        print(json.dumps({"cpu_baseline": cpu_baseline(args.grid, args.sim_steps, args.checkpoint, args.cpu_sample_steps,
+                                                       args.cpu_threads)}), flush=True)
         return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (the hot path is single-GPU here); got --gpus {args.gpus}", file=sys.stderr)
@@ -386,7 +394,7 @@ def main():
         "kernels": {k: {kk: (round(vv, 2) if isinstance(vv, float) else vv) for kk, vv in v.items()} for k, v in kern.items()},
     }
     if not args.no_cpu_baseline:
-        cb = cpu_baseline(args.grid, args.sim_steps, args.checkpoint, args.cpu_sample_steps)
+        cb = cpu_baseline(args.grid, args.sim_steps, args.checkpoint, args.cpu_sample_steps, args.cpu_threads)
         if "none" in results:
             # the GPU from x0 = 0, as the oracle starts every solve; the oracle iterates Jacobi-PCG,
             # the GPU's default PCG form is preconditioned by a multigrid V-cycle, so the ratio holds
