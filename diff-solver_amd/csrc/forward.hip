@@ -94,7 +94,7 @@ __global__ __launch_bounds__(256) void integrate_kernel(Geo g, const uint8_t* __
 // advect_velocity_forward_kernel (operators.cu:381-388), optionally with the constraint fused:
 // constrained faces skip the trace; a periodic duplicate face traces face 0's position.
 template <bool Constrain, bool HasSolids>
-__global__ __launch_bounds__(256) void advect_velocity_kernel(Geo g, const VBound* __restrict__ bp, const uint8_t* __restrict__ mask,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void advect_velocity_kernel(Geo g, const VBound* __restrict__ bp, const uint8_t* __restrict__ mask,
                                                               Vel3 u, CF3 collider, F3 out) {
     const VBound& b = *bp;
     const uint32_t fi = xcd_block() * blockDim.x + threadIdx.x;
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(256) void project_kernel(Geo g, const uint8_t* __re
 
 // advect_scalar_forward_kernel (operators.cu:499-507) for NF fields sharing one RK2 trace.
 template <int NF, bool HasSolids>
-__global__ __launch_bounds__(256) void advect_scalar_kernel(Geo g, const VBound* __restrict__ vbp, const SBound* __restrict__ sb0p,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void advect_scalar_kernel(Geo g, const VBound* __restrict__ vbp, const SBound* __restrict__ sb0p,
                                                             const SBound* __restrict__ sb1p, const uint8_t* __restrict__ mask,
                                                             const float* __restrict__ col0, const float* __restrict__ col1,
                                                             const float* __restrict__ src0, const float* __restrict__ src1, Vel3 u,
