@@ -142,7 +142,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void a
 // move a few cells at most, so their 56 scatters per face land in a small box around the brick.
 // Scatters inside the box (brick plus a halo of kVjpHalo faces, each component) are LDS atomics;
 // those outside it (long traces, periodic wraps) stay global atomics. The box is flushed once.
-constexpr int kVjpBX = 32, kVjpBY = 8, kVjpBZ = 4, kVjpHalo = 2;
+constexpr int kVjpBX = 32, kVjpBY = 8, kVjpBZ = 4, kVjpHalo = 1;
 constexpr int kWinX = kVjpBX + 2 * kVjpHalo + 1, kWinY = kVjpBY + 2 * kVjpHalo + 1, kWinZ = kVjpBZ + 2 * kVjpHalo + 1;
 constexpr int kWin = kWinX * kWinY * kWinZ;   // per component
 
@@ -187,7 +187,7 @@ __device__ __forceinline__ bool start_gatherable(const Geo& g, uint32_t per, int
 }
 
 template <int A, bool HasSolids, bool GatherStart>
-__global__ __launch_bounds__(256) void advect_velocity_vjp_kernel(Geo g, const VBound* __restrict__ bp, const uint8_t* __restrict__ mask,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void advect_velocity_vjp_kernel(Geo g, const VBound* __restrict__ bp, const uint8_t* __restrict__ mask,
                                                                   Vel3 u, const float* __restrict__ oa, F3 va, F3 start, uint32_t per) {
     __shared__ float win[3 * kWin];
     const VBound& b = *bp;
