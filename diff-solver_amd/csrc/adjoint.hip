@@ -76,7 +76,7 @@ struct ScalarBrickSink {
 };
 
 template <int NF, bool HasSolids>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void advect_scalar_vjp_kernel(Geo g, const VBound* __restrict__ vbp, const SBound* __restrict__ sb0p,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void advect_scalar_vjp_kernel(Geo g, const VBound* __restrict__ vbp, const SBound* __restrict__ sb0p,
                                                                 const SBound* __restrict__ sb1p, const uint8_t* __restrict__ mask,
                                                                 const float* __restrict__ src0, const float* __restrict__ src1, Vel3 u,
                                                                 const float* __restrict__ oa0, const float* __restrict__ oa1,
