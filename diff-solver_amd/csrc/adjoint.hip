@@ -663,18 +663,18 @@ void launch_force_source_vjp(ds_context* c, CF3 forced_adj, const float* rho_s, 
         require(prm.confinement != nullptr, "parameters: vorticity_confinement must be a device pointer with confinement on");
         const VortReverse vr{VortFields{{c->vort_w[0], c->vort_w[1], c->vort_w[2]}, c->vort_mag},
                              F3{{c->vort_ga[0], c->vort_ga[1], c->vort_ga[2]}}, F3{{c->vort_t[0], c->vort_t[1], c->vort_t[2]}}};
-        force_source_vjp_kernel<true><<<kPcgBlocks, kBlock, 0, c->stream>>>(c->geo, c->mask, forced_adj, rho_s, t_s, rho_s_adj, t_s_adj,
+        force_source_vjp_kernel<true><<<kForceBlocks, kBlock, 0, c->stream>>>(c->geo, c->mask, forced_adj, rho_s, t_s, rho_s_adj, t_s_adj,
                                                                             prm, e_adj, prev_rho, prev_t, s_rho_adj, s_t_adj, vr,
                                                                             c->param_partials);
     } else {
-        force_source_vjp_kernel<false><<<kPcgBlocks, kBlock, 0, c->stream>>>(c->geo, c->mask, forced_adj, rho_s, t_s, rho_s_adj, t_s_adj,
+        force_source_vjp_kernel<false><<<kForceBlocks, kBlock, 0, c->stream>>>(c->geo, c->mask, forced_adj, rho_s, t_s, rho_s_adj, t_s_adj,
                                                                              prm, e_adj, prev_rho, prev_t, s_rho_adj, s_t_adj, VortReverse{},
                                                                              c->param_partials);
     }
     check_launch("force_source_vjp_kernel");
     if (param_adj) {
-        if (c->vorticity) param_finalize_kernel<4><<<1, kBlock, 0, c->stream>>>(c->param_partials, kPcgBlocks, param_out(param_adj));
-        else param_finalize_kernel<3><<<1, kBlock, 0, c->stream>>>(c->param_partials, kPcgBlocks, param_out(param_adj));
+        if (c->vorticity) param_finalize_kernel<4><<<1, kBlock, 0, c->stream>>>(c->param_partials, kForceBlocks, param_out(param_adj));
+        else param_finalize_kernel<3><<<1, kBlock, 0, c->stream>>>(c->param_partials, kForceBlocks, param_out(param_adj));
         check_launch("param_finalize_kernel");
     }
 }

@@ -489,7 +489,7 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
             }
             c->lambda_hi = device_alloc<float>(c, n);
             c->lambda_lo = device_alloc<float>(c, n);
-            c->param_partials = device_alloc<double>(c, 4 * (size_t)kPcgBlocks);
+            c->param_partials = device_alloc<double>(c, 4 * (size_t)std::max(kPcgBlocks, kForceBlocks));
             for (int k = 0; k < 2; ++k) c->sourced_t[k] = device_alloc<float>(c, n);
             for (int a = 0; a < 3; ++a) {
                 c->force_t[a] = device_alloc<float>(c, n);

@@ -75,6 +75,9 @@ inline uint32_t xcd_blocks_for(uint64_t count, uint32_t block = kBlock) { return
 // Reduction layout: kernels of kPcgBlocks (256-thread) blocks leave one FP64 partial per block
 // per quantity; consumers finish the sum in a fixed order (deterministic, no atomics).
 constexpr uint32_t kPcgBlocks = 512;
+// Blocks of the force and source reverse (per-cell work with FP64 parameter partials): enough waves
+// to keep its dependent gathers in flight.
+constexpr uint32_t kForceBlocks = 2048;
 // Blocks of the example metric reduction (one partial per block per metric, finished in order).
 constexpr uint32_t kMetricBlocks = 512;
 // PCG iteration kernels: 1024-thread blocks, each over a contiguous range of whole multiples of
@@ -282,7 +285,7 @@ struct ds_context {
     float* lambda_hi = nullptr;       // adjoint pressure solve solution (hi + lo)
     float* lambda_lo = nullptr;
     bool lambda_valid = false;
-    double* param_partials = nullptr; // 4 * kPcgBlocks
+    double* param_partials = nullptr; // 4 * kForceBlocks
     double* metric_partials = nullptr; // 6 * kMetricBlocks (ds_example_reduce_metrics)
 
     // forward-mode (tangent) scratch (device, differentiable mode)
