@@ -463,6 +463,13 @@ ds_status ds_context_create(const ds_config* config, uint32_t mode, void* stream
             c->pcg.blocks = (uint32_t)((G + 7) & ~7ull);
             // the fused kernel gives each thread at most one producer partial
             if (c->pcg.blocks > kPcgThreads) throw Failure{DS_ERROR_UNSUPPORTED, "grid too large for the PCG tiling"};
+            // the multigrid form has no fused kernel: its reducing kernels (and the set-up, true
+            // residual and finalisation that share their partial slots) take one block per 2048
+            // cells, up to the slots' capacity, for more waves in flight
+            if (cfg.cg_variant == DS_CG_MULTIGRID) {
+                const uint64_t b = std::min<uint64_t>(kPcgMaxBlocks, std::max<uint64_t>(8, ((uint64_t)g.n + 2047) / 2048));
+                c->pcg.blocks = (uint32_t)((b + 7) & ~7ull);
+            }
         }
         c->pcg.partials = device_alloc<double>(c, 2 * (size_t)kPartCount * kPcgMaxBlocks);   // two sets
         c->pcg.hist = device_alloc<double>(c, 2 * ((size_t)c->max_iterations + 4));
