@@ -75,6 +75,8 @@ struct ScalarBrickSink {
     }
 };
 
+// 4 waves per SIMD (registers capped at 128, the rest spilled), the most its 38.5 KB of LDS allows:
+// the dependent gathers of the trace want waves more than registers.
 template <int NF, bool HasSolids>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void advect_scalar_vjp_kernel(Geo g, const VBound* __restrict__ vbp, const SBound* __restrict__ sb0p,
                                                                 const SBound* __restrict__ sb1p, const uint8_t* __restrict__ mask,
@@ -186,6 +188,7 @@ __device__ __forceinline__ bool start_gatherable(const Geo& g, uint32_t per, int
     return ok;
 }
 
+// 4 waves per SIMD (registers capped at 128, the rest spilled), the most its 32 KB window allows.
 template <int A, bool HasSolids, bool GatherStart>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void advect_velocity_vjp_kernel(Geo g, const VBound* __restrict__ bp, const uint8_t* __restrict__ mask,
                                                                   Vel3 u, const float* __restrict__ oa, F3 va, F3 start, uint32_t per) {

@@ -93,6 +93,7 @@ __global__ __launch_bounds__(256) void integrate_kernel(Geo g, const uint8_t* __
 
 // advect_velocity_forward_kernel (operators.cu:381-388), optionally with the constraint fused:
 // constrained faces skip the trace; a periodic duplicate face traces face 0's position.
+// 8 waves per SIMD (63 registers, nothing spilled): the trace's dependent gathers want waves.
 template <bool Constrain, bool HasSolids>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void advect_velocity_kernel(Geo g, const VBound* __restrict__ bp, const uint8_t* __restrict__ mask,
                                                               Vel3 u, CF3 collider, F3 out) {
@@ -170,6 +171,7 @@ __global__ __launch_bounds__(256) void project_kernel(Geo g, const uint8_t* __re
 }
 
 // advect_scalar_forward_kernel (operators.cu:499-507) for NF fields sharing one RK2 trace.
+// 8 waves per SIMD, as advect_velocity_kernel.
 template <int NF, bool HasSolids>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void advect_scalar_kernel(Geo g, const VBound* __restrict__ vbp, const SBound* __restrict__ sb0p,
                                                             const SBound* __restrict__ sb1p, const uint8_t* __restrict__ mask,
