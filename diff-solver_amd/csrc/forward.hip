@@ -77,17 +77,19 @@ __global__ __launch_bounds__(256) void integrate_kernel(Geo g, const uint8_t* __
         decode_face_axis<A>(g, local, x, y, z);
         const int coord = A == 0 ? x : A == 1 ? y : z;
         const int n_a = dim_size(g, A);
-        float sum = 0.0f, count = 0.0f;
         const float* fa = f.p[A];
-        if (coord > 0) {
-            const uint32_t c0 = cidx(g, x - (A == 0), y - (A == 1), z - (A == 2));
-            if (mask[c0] == 0) { sum += fa[c0]; count += 1.0f; }
-        }
-        if (coord < n_a) {
-            const uint32_t c1 = cidx(g, x, y, z);
-            if (mask[c1] == 0) { sum += fa[c1]; count += 1.0f; }
-        }
-        out.p[A][local] = u.p[A][local] + g.dt * (count == 0.0f ? 0.0f : sum / count);
+        // both neighbour cells' flag and force loaded at once: a side outside the domain reads the
+        // other side's cell and is not counted
+        const bool h0 = coord > 0, h1 = coord < n_a;
+        const uint32_t cl = cidx(g, x - (A == 0 && h0), y - (A == 1 && h0), z - (A == 2 && h0));   // low side, else the cell
+        const uint32_t cc = h1 ? cidx(g, x, y, z) : cl;
+        const uint32_t c0 = h0 ? cl : cc;
+        const uint8_t m0 = mask[c0], m1 = mask[cc];
+        const float f0 = fa[c0], f1 = fa[cc], ua = u.p[A][local];
+        float sum = 0.0f, count = 0.0f;
+        if (h0 && m0 == 0) { sum += f0; count += 1.0f; }
+        if (h1 && m1 == 0) { sum += f1; count += 1.0f; }
+        out.p[A][local] = ua + g.dt * (count == 0.0f ? 0.0f : sum / count);
     });
 }
 
@@ -153,19 +155,20 @@ __global__ __launch_bounds__(256) void project_kernel(Geo g, const uint8_t* __re
         const int coord = A == 0 ? x : A == 1 ? y : z;
         const int n_a = dim_size(g, A);
         const float ua = u.p[A][local];
-        if (coord == 0 || coord == n_a) {
-            out.p[A][local] = ua;
-            return;
-        }
-        const uint32_t c1 = cidx(g, x, y, z);
+        // flags and pressures of both cells loaded at once (a domain face reads cell 0 and keeps ua)
+        const bool edge = coord == 0 || coord == n_a;
         const uint32_t stride = A == 0 ? 1u : A == 1 ? (uint32_t)g.nx : (uint32_t)g.nx * (uint32_t)g.ny;
-        const uint32_t c0 = c1 - stride;
-        if (mask[c0] != 0 || mask[c1] != 0) {
+        const uint32_t c1 = edge ? 0u : cidx(g, x, y, z);
+        const uint32_t c0 = edge ? 0u : c1 - stride;
+        const uint8_t m0 = mask[c0], m1 = mask[c1];
+        const float p0 = ph[c0], p1 = ph[c1];
+        const float l0 = pl ? pl[c0] : 0.0f, l1 = pl ? pl[c1] : 0.0f;
+        if (edge || m0 != 0 || m1 != 0) {
             out.p[A][local] = ua;
             return;
         }
-        float dp = ph[c1] - ph[c0];
-        if (pl) dp += pl[c1] - pl[c0];
+        float dp = p1 - p0;
+        if (pl) dp += l1 - l0;
         out.p[A][local] = ua - g.dt * dp / g.h;
     });
 }
