@@ -32,16 +32,17 @@ __global__ __launch_bounds__(256) void base_residual_kernel(Geo geo, uint32_t pe
                                                            const float* __restrict__ b, const float* __restrict__ xb,
                                                            float* __restrict__ r) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < geo.n; i += gridDim.x * blockDim.x) {
+        // every code and value is loaded up front (the neighbours are cells of the grid), the codes
+        // select afterwards, so the loads issue together
         const uint32_t d = code[i];
-        if (!d) {
-            r[i] = 0.0f;
-            continue;
-        }
         int x, y, z;
         decode_cell(geo, i, x, y, z);
         const int c[3]{x, y, z}, ext[3]{geo.nx, geo.ny, geo.nz};
         double s = (double)d * (double)xb[i];
+        const float bi = b[i];
+#pragma unroll
         for (int a = 0; a < 3; ++a)
+#pragma unroll
             for (int dir = -1; dir <= 1; dir += 2) {
                 int q[3]{c[0], c[1], c[2]};
                 q[a] += dir;
@@ -50,9 +51,10 @@ __global__ __launch_bounds__(256) void base_residual_kernel(Geo geo, uint32_t pe
                     q[a] = q[a] < 0 ? ext[a] - 1 : 0;
                 }
                 const uint32_t j = cidx(geo, q[0], q[1], q[2]);
-                if (code[j]) s -= (double)xb[j];
+                const double xj = (double)xb[j];
+                s -= code[j] ? xj : 0.0;
             }
-        r[i] = (float)((double)b[i] - s);
+        r[i] = d ? (float)((double)bi - s) : 0.0f;
     }
 }
 
