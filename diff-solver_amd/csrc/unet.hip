@@ -155,12 +155,13 @@ __global__ __launch_bounds__(256) void step_partials_kernel(Geo geo, uint32_t pe
     }
 #pragma unroll
     for (int k = 0; k < kStepCells; ++k) {
-        if (!cd[k]) continue;
-        const uint32_t i = i0 + k * blockDim.x;
+        // the loads of a cell outside the grid read its last cell; its code 0 drops the terms
+        const uint32_t i = min(i0 + k * blockDim.x, geo.n - 1u);
         int x, y, z;
         decode_cell(geo, i, x, y, z);
         const int c[3]{x, y, z}, ext[3]{geo.nx, geo.ny, geo.nz};
         const double di = d[i];
+        const float ri = r[i];
         double s = (double)cd[k] * di;
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -174,8 +175,10 @@ __global__ __launch_bounds__(256) void step_partials_kernel(Geo geo, uint32_t pe
                 }
                 s -= (double)d[cidx(geo, q[0], q[1], q[2])];
             }
-        v[0] += di * (double)r[i];
-        v[1] += di * s;
+        if (cd[k]) {
+            v[0] += di * (double)ri;
+            v[1] += di * s;
+        }
     }
     block_sum<2>(v, lds);
     if (threadIdx.x == 0) {
